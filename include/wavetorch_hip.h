@@ -1,0 +1,210 @@
+/*
+ * wavetorch_hip.h -- C ABI of the MI355X (gfx950) WaveRNN time-loop library
+ * (libwavetorch_hip.so).
+ *
+ * The reference (fancompute/wavetorch v0.2.1) is pure Python/PyTorch; its hot
+ * path is the per-step torch-op sequence driven from rnn.py.  Each entry point
+ * below replaces a named piece of that path; the citation on each declaration
+ * gives the reference file:line it stands in for.  The Python package
+ * wavetorch_amd binds these with ctypes (see wavetorch_amd/_lib.py) behind the
+ * reference's own nn.Module API.
+ *
+ * Conventions
+ *   - All tensors are caller-owned device memory, fp32, C-contiguous:
+ *     fields (B, Nx, Ny) with Ny fastest; signals (B, T); probe series
+ *     (B, T, ntap).  The library never frees caller memory.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Every call is asynchronous on that stream unless stated otherwise.
+ *   - Every function returns an int status: 0 on success, a negative
+ *     wt_status code otherwise; it never throws or aborts.  The message of
+ *     the last failure on the calling thread is available from
+ *     wt_last_error().
+ *   - Reentrant: no global mutable state except the per-thread error string
+ *     and the per-device kernel-attribute cache (mutex protected).
+ */
+#ifndef WAVETORCH_HIP_H
+#define WAVETORCH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WT_ABI_VERSION 1
+
+enum wt_status {
+  WT_OK = 0,
+  WT_EINVAL = -1,       /* bad argument: null pointer, bad shape, bad index */
+  WT_ECFL = -2,         /* dt violates the CFL bound (cell.py:66-73) */
+  WT_EHIP = -3,         /* a HIP runtime call failed */
+  WT_EUNSUPPORTED = -4, /* shape/plan not supported by any kernel variant */
+  WT_ENOMEM = -5,       /* workspace too small */
+};
+
+/* Version of this ABI (WT_ABI_VERSION). */
+int wt_abi_version(void);
+
+/* Copies the last error message of the calling thread into buf (always
+ * NUL-terminated when len > 0) and returns its full length. */
+int wt_last_error(char *buf, size_t len);
+
+/* ------------------------------------------------------------------------ */
+/* Material coefficients                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Leapfrog coefficients of cell.py:12-17, with the reference's fp32 rounding:
+ *   dtm2 = 1/(dt*dt), dtm1 = 1/dt
+ *   A  = 1/(dtm2 + b*dtm1)      ((dt**-2 + b*dt**-1).pow(-1), cell.py:13)
+ *   Bm = dtm2 - b*dtm1          (cell.py:14)
+ *   C2 = c*c                    (c.pow(2), cell.py:15)
+ * over n cells.  Replaces the six material-field ops the reference
+ * recomputes on every step (cell.py:13-15). */
+int wt_coef_linear(const float *b, const float *c, int64_t n, float dt,
+                   float *A, float *Bm, float *C2, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Single step: backs cell.TimeStep (cell.py:20-44)                          */
+/* ------------------------------------------------------------------------ */
+
+/* y = _time_step(b, c, y1, y2, dt, h)  (cell.py:12-17, operators.py:5-11).
+ * b and c are (Nx,Ny) when bc_per_sample == 0, else (B,Nx,Ny).
+ * Bitwise equal to the reference's fp32 CPU result for h == 1. */
+int wt_step_fwd(const float *b, const float *c, int32_t bc_per_sample,
+                const float *y1, const float *y2, int32_t B, int32_t Nx,
+                int32_t Ny, float dt, float h, float *y, void *stream);
+
+/* TimeStep.backward (cell.py:26-44) for grad_output g.  Any output pointer
+ * may be NULL (the needs_input_grad gating of cell.py:32,35,37,41).
+ * grad_b / grad_c are (Nx,Ny) summed over the batch when bc_per_sample == 0
+ * (what autograd's reduction to the input shape yields), else (B,Nx,Ny). */
+int wt_step_bwd(const float *b, const float *c, int32_t bc_per_sample,
+                const float *y1, const float *y2, const float *g, int32_t B,
+                int32_t Nx, int32_t Ny, float dt, float h, float *grad_b,
+                float *grad_c, float *grad_y1, float *grad_y2, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Whole time loop: backs WaveRNN.forward (rnn.py:21-72) and the autograd     */
+/* replay of its tape (cell.py:26-44 per step)                               */
+/* ------------------------------------------------------------------------ */
+
+/* Static description of one WaveRNN configuration.  Host memory. */
+typedef struct wt_config {
+  int32_t Nx, Ny;      /* geom.domain_shape (geom.py:19) */
+  float dt, h;         /* cell.dt (cell.py:60), geom.h (geom.py:20) */
+  /* sources (source.py:7-22): nsrc_px rows of (x, y, source_index); each
+   * source's pixels deduplicated (index_put assigns, source.py:20).  Every
+   * source receives the same input sample x[b, n] (rnn.py:56-57), with
+   * scale dt**2 = 1.0 (source.py:15). */
+  int32_t nsrc_px;
+  const int32_t *src_px;
+  /* probes (probe.py:6-27): ntap rows of (x, y, kind); kind 0 = WaveProbe
+   * (u), kind 1 = WaveIntensityProbe (u*u).  Output column = row index. */
+  int32_t ntap;
+  const int32_t *taps;
+  /* nonlinear cell (cell.py:8-9, 94-102): active when sat_b0 > 0 or
+   * c_nl != 0.  c_clamp != 0 clamps c to [c_min, c_max] (an extension). */
+  float sat_b0, sat_uth, c_nl;
+  int32_t c_clamp;
+  float c_min, c_max;
+} wt_config;
+
+/* Kernel-variant / tiling choice.  All fields 0 = let the library choose
+ * (heuristic); wavetorch_amd.autotune fills these from measurements. */
+typedef struct wt_tuning {
+  int32_t plain;     /* 1 = one launch per step (the plain GPU form) */
+  int32_t strips;    /* row strips per sample (tiled form) */
+  int32_t tblock;    /* time steps per launch (temporal-block depth) */
+  int32_t nt;        /* threads per workgroup: 256, 512 or 1024 */
+  int32_t cpt;       /* cells per thread (compile-time variants) */
+  int32_t ck_every;  /* checkpoint interval K in steps (0 = 64) */
+} wt_tuning;
+
+/* Creates a plan for `cfg` at batch B and length T: validates the
+ * configuration (indices, CFL as cell.py:66-73 with cmax from `cmax`),
+ * picks the kernel variant and uploads the source/probe/tile tables to the
+ * current device (synchronously).  The plan is library-owned; destroy with
+ * wt_plan_destroy.  `tuning` may be NULL. */
+int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax,
+                   const wt_tuning *tuning, void **plan_out);
+int wt_plan_destroy(void *plan);
+/* The tuning actually chosen (after heuristics). */
+int wt_plan_tuning(const void *plan, wt_tuning *out);
+
+/* Bytes of the two caller-provided workspaces:
+ *   saved: lives from wt_loop_fwd to wt_loop_adj (coefficients, checkpoints,
+ *          probe amplitudes); needed only when with_grad.
+ *   temp:  scratch of one call (state ping-pong, adjoint segment buffer,
+ *          adjoint state, gradient partials). */
+int wt_workspace_bytes(const void *plan, int32_t with_grad, int64_t *saved,
+                       int64_t *temp);
+
+/* Forward time loop (rnn.py:21-72).
+ *   b, c        (Nx,Ny) geom.b and c_linear (rnn.py:46)
+ *   rho         (Nx,Ny) raw density (rnn.py:47); used by the nonlinear cell
+ *   x           (B,T) input signal
+ *   y           (B,T,ntap) probe output (rnn.py:59-64,70), may be NULL
+ *   fields      (B,T/field_stride,Nx,Ny) fields u_{n+1} for n+1 a multiple of
+ *               field_stride (rnn.py:66 with output_fields; stride 1 =
+ *               reference behaviour), may be NULL
+ *   with_grad   store checkpoints/amplitudes into `saved` for wt_loop_adj */
+int wt_loop_fwd(void *plan, const float *b, const float *c, const float *rho,
+                const float *x, float *y, float *fields, int32_t field_stride,
+                int32_t with_grad, void *saved, void *temp, void *stream);
+
+/* Reverse sweep (the autograd replay of cell.py:26-44 over rnn.py:50-70)
+ * with checkpoint recompute every K steps.
+ *   ybar        (B,T,ntap) dJ/dy, or NULL
+ *   fbar        (B,T,Nx,Ny) dJ/d(fields) (stride-1 fields only), or NULL
+ *   grad_c      (Nx,Ny) dJ/dc_linear, summed over batch, or NULL
+ *   grad_b      (Nx,Ny) dJ/db (nonlinear cell only), or NULL
+ *   grad_rho    (Nx,Ny) dJ/drho through the nonlinear terms, or NULL
+ *   grad_x      (B,T)   dJ/dx, or NULL
+ * Output buffers are overwritten (not accumulated). */
+int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho,
+                const float *x, const float *ybar, const float *fbar,
+                void *saved, void *temp, float *grad_c, float *grad_b,
+                float *grad_rho, float *grad_x, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Geometry parameterisation (geom.py:207-233)                               */
+/* ------------------------------------------------------------------------ */
+
+/* c = c0 + (c1-c0) * P(Blur^N(rho))   (WaveGeometryFreeForm.c, geom.py:231-233)
+ *   kernel  (2r+1)^2 normalised blur kernel (geom.py:149-154)
+ *   P(x)  = (tanh(beta*eta) + tanh(beta*(x-eta))) / (tanh(beta*eta)+tanh(beta*(1-eta)))
+ *   work    2*Nx*Ny floats of device scratch (blur ping-pong)
+ *   rho_proj (Nx,Ny) receives P(Blur^N(rho)) (may be NULL)
+ *   blur_out (Nx,Ny) receives Blur^N(rho) (needed by the backward; may be NULL) */
+int wt_geom_freeform_c(const float *rho, const float *kernel, int32_t radius,
+                       int32_t blur_N, int32_t Nx, int32_t Ny, float eta,
+                       float beta, float c0, float c1, float *c,
+                       float *rho_proj, float *blur_out, float *work,
+                       void *stream);
+
+/* Backward of wt_geom_freeform_c: grad_rho = (Blur^T)^N((c1-c0) P'(blur) grad_c).
+ *   blur_in (Nx,Ny) Blur^N(rho) from the forward; work 2*Nx*Ny floats. */
+int wt_geom_freeform_c_bwd(const float *grad_c, const float *blur_in,
+                           const float *kernel, int32_t radius, int32_t blur_N,
+                           int32_t Nx, int32_t Ny, float eta, float beta,
+                           float c0, float c1, float *grad_rho, float *work,
+                           void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* HIP graph capture of any sequence of the calls above                      */
+/* ------------------------------------------------------------------------ */
+
+/* Begin stream capture on `stream` (hipStreamCaptureModeThreadLocal).  Every
+ * wt_* call made on that stream until wt_graph_end is recorded (the library
+ * records no memset nodes). */
+int wt_graph_begin(void *stream);
+/* End capture and instantiate; *graph_out is a library-owned handle. */
+int wt_graph_end(void *stream, void **graph_out);
+int wt_graph_launch(void *graph, void *stream);
+int wt_graph_destroy(void *graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAVETORCH_HIP_H */

@@ -1,0 +1,25 @@
+"""pytest configuration: the ``gpu`` marker and shared fixtures.
+
+Tests marked ``gpu`` need a HIP device and the built library; everything
+else runs on the CPU (oracle checks, host logic, ABI symbol checks).
+"""
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X) and the built library")
+
+
+@pytest.fixture(scope="session")
+def cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch.device("cuda:0")

@@ -1,0 +1,159 @@
+"""The fused time loop behind ``WaveRNN.forward`` (reference: rnn.py:21-72).
+
+``WaveLoop`` is one ``torch.autograd.Function`` for the whole T-step loop:
+its forward runs ``wt_loop_fwd`` (leapfrog + sources + probes, checkpoints
+every K steps) and its backward runs ``wt_loop_adj`` (segment recompute from
+checkpoints + hand-written adjoint), replacing the per-step autograd tape the
+reference builds (cell.py:20-44 replayed T times).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+@dataclass(frozen=True)
+class LoopConfig:
+    """Host-side description of one WaveRNN configuration (``wt_config``)."""
+    Nx: int
+    Ny: int
+    dt: float
+    h: float
+    src_px: tuple          # ((x, y, source_index), ...)
+    taps: tuple            # ((x, y, kind), ...), output column = position
+    sat_b0: float = 0.0
+    sat_uth: float = 0.0
+    c_nl: float = 0.0
+    c_clamp: tuple | None = None
+
+    @property
+    def nonlinear(self) -> bool:
+        return self.sat_b0 > 0 or self.c_nl != 0
+
+    def to_c(self):
+        src = np.ascontiguousarray(np.asarray(self.src_px, dtype=np.int32).reshape(-1, 3))
+        taps = np.ascontiguousarray(np.asarray(self.taps, dtype=np.int32).reshape(-1, 3))
+        cfg = _lib.WtConfig()
+        cfg.Nx, cfg.Ny = self.Nx, self.Ny
+        cfg.dt, cfg.h = self.dt, self.h
+        cfg.nsrc_px = len(src)
+        cfg.src_px = src.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        cfg.ntap = len(taps)
+        cfg.taps = taps.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        cfg.sat_b0, cfg.sat_uth, cfg.c_nl = self.sat_b0, self.sat_uth, self.c_nl
+        if self.c_clamp is not None:
+            cfg.c_clamp, cfg.c_min, cfg.c_max = 1, self.c_clamp[0], self.c_clamp[1]
+        return cfg, (src, taps)  # keep the arrays alive while cfg is used
+
+
+class Plan:
+    """Owner of a library plan (validated config + device tables + tiling)."""
+
+    def __init__(self, cfg: LoopConfig, B: int, T: int, cmax: float, device, tuning=None):
+        self.cfg, self.B, self.T, self.device = cfg, B, T, device
+        c, keep = cfg.to_c()
+        tun = None
+        if tuning:
+            tun = _lib.WtTuning(**{k: int(v) for k, v in tuning.items()})
+        handle = ctypes.c_void_p()
+        L = _lib.lib()
+        with torch.cuda.device(device):
+            _lib.check(L.wt_plan_create(ctypes.byref(c), B, T, float(cmax),
+                                        ctypes.byref(tun) if tun is not None else None,
+                                        ctypes.byref(handle)), "wt_plan_create")
+        self.handle = handle
+        del keep
+        t = _lib.WtTuning()
+        _lib.check(L.wt_plan_tuning(handle, ctypes.byref(t)), "wt_plan_tuning")
+        self.tuning = t.as_dict()
+        s0, t0 = _lib.c_i64(), _lib.c_i64()
+        _lib.check(L.wt_workspace_bytes(handle, 1, ctypes.byref(s0), ctypes.byref(t0)),
+                   "wt_workspace_bytes")
+        self.saved_bytes, self.temp_bytes = s0.value, t0.value
+        _lib.check(L.wt_workspace_bytes(handle, 0, ctypes.byref(s0), ctypes.byref(t0)),
+                   "wt_workspace_bytes")
+        self.temp_bytes_nograd = t0.value
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().wt_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+def _buf(nbytes, device):
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+class WaveLoop(torch.autograd.Function):
+    """y = WaveRNN time loop (x, c, b, rho); backward = checkpointed adjoint."""
+
+    @staticmethod
+    def forward(ctx, x, c, b, rho, plan: Plan, output_fields: bool, field_stride: int):
+        dev = x.device
+        x = x.contiguous()
+        c = c.contiguous()
+        b = b.contiguous()
+        rho_c = rho.contiguous() if rho is not None else None
+        B, T = x.shape
+        Nx, Ny = plan.cfg.Nx, plan.cfg.Ny
+        need = ctx.needs_input_grad
+        with_grad = bool(need[0] or need[1] or need[2] or (rho is not None and need[3]))
+        ntap = len(plan.cfg.taps)
+        y = None
+        fields = None
+        if output_fields:
+            fields = torch.empty((B, T // field_stride, Nx, Ny), dtype=torch.float32, device=dev)
+        else:
+            y = torch.empty((B, T, ntap), dtype=torch.float32, device=dev)
+        saved = _buf(plan.saved_bytes, dev) if with_grad else None
+        temp = _buf(plan.temp_bytes if with_grad else plan.temp_bytes_nograd, dev)
+        s = _lib.stream_ptr(dev)
+        _lib.check(_lib.lib().wt_loop_fwd(
+            plan.handle, b.data_ptr(), c.data_ptr(), _lib.ptr(rho_c), x.data_ptr(), _lib.ptr(y),
+            _lib.ptr(fields), int(field_stride), int(with_grad), _lib.ptr(saved),
+            temp.data_ptr(), s), "wt_loop_fwd")
+        del temp
+        if with_grad:
+            ctx.save_for_backward(x, c, b, rho_c if rho_c is not None else x.new_empty(0))
+            ctx.ws = saved
+            ctx.plan = plan
+            ctx.has_rho = rho_c is not None
+            ctx.output_fields = output_fields
+            ctx.field_stride = field_stride
+        return fields if output_fields else y
+
+    @staticmethod
+    def backward(ctx, gout):
+        x, c, b, rho = ctx.saved_tensors
+        plan = ctx.plan
+        if ctx.output_fields and ctx.field_stride != 1:
+            raise NotImplementedError("wavetorch_amd: gradients through strided field snapshots")
+        rho = rho if ctx.has_rho else None
+        dev = x.device
+        need = ctx.needs_input_grad
+        Nx, Ny = plan.cfg.Nx, plan.cfg.Ny
+        g = gout.contiguous()
+        gc = torch.empty((Nx, Ny), dtype=torch.float32, device=dev) if need[1] else None
+        gb = torch.empty((Nx, Ny), dtype=torch.float32, device=dev) if need[2] else None
+        grho = torch.empty((Nx, Ny), dtype=torch.float32, device=dev) \
+            if (rho is not None and need[3]) else None
+        gx = torch.empty_like(x) if need[0] else None
+        temp = _buf(plan.temp_bytes, dev)
+        s = _lib.stream_ptr(dev)
+        _lib.check(_lib.lib().wt_loop_adj(
+            plan.handle, b.data_ptr(), c.data_ptr(), _lib.ptr(rho), x.data_ptr(),
+            None if ctx.output_fields else g.data_ptr(),
+            g.data_ptr() if ctx.output_fields else None,
+            ctx.ws.data_ptr(), temp.data_ptr(), _lib.ptr(gc), _lib.ptr(gb), _lib.ptr(grho),
+            _lib.ptr(gx), s), "wt_loop_adj")
+        ctx.ws = None
+        return gx, gc, gb, grho, None, None, None

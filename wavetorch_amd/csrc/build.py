@@ -1,0 +1,70 @@
+"""Builds libwavetorch_hip.so for gfx950 in-tree with hipcc.
+
+Usage: ``python -m wavetorch_amd.csrc.build`` (also called by
+``__graft_entry__.build()``).  Every translation unit is compiled with
+
+* ``--offload-arch=gfx950``               MI355X only
+* ``-ffp-contract=off``                   no implicit FMA: the forward kernels
+                                          reproduce the reference's per-op
+                                          fp32 rounding; FMAs in the adjoint
+                                          are explicit ``fmaf`` calls
+* ``-Xclang -target-feature -Xclang -packed-fp32-ops``
+                                          no packed-FP32 VALU instructions
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+OUT = os.path.join(PKG, "_lib")
+LIBNAME = "libwavetorch_hip.so"
+SOURCES = ["wt_runtime.hip", "wt_plain.hip", "wt_tiled.hip", "wt_geom.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-Wno-unused-command-line-argument",
+]
+
+
+def _compile(src: str, obj: str, verbose: bool) -> None:
+    cmd = [HIPCC, *FLAGS, "-c", os.path.join(HERE, src), "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msg = "\n".join(line for line in (r.stdout + r.stderr).splitlines()
+                    if "packed-fp32-ops" not in line)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{msg}")
+    if verbose and msg.strip():
+        print(msg)
+
+
+def build(verbose: bool = False, jobs: int = 4) -> str:
+    """Compiles every kernel source and links the shared library; returns its path."""
+    os.makedirs(OUT, exist_ok=True)
+    objdir = os.path.join(OUT, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, s.replace(".hip", ".o")) for s in SOURCES]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = [ex.submit(_compile, s, o, verbose) for s, o in zip(SOURCES, objs)]
+        for f in futs:
+            f.result()
+    lib = os.path.join(OUT, LIBNAME)
+    tmp = lib + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, lib)
+    return lib
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,111 @@
+// wt_internal.h -- shared definitions of the gfx950 WaveRNN kernels and the
+// host runtime of libwavetorch_hip.so.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/wavetorch_hip.h"
+
+namespace wt {
+
+// ---------------------------------------------------------------- tables
+//
+// Special pixels: every pixel that carries a source and/or a probe tap,
+// deduplicated, sorted by (x, y).  One row of 8 int32:
+//   [0] x  [1] y  [2] source multiplicity (number of sources containing it)
+//   [3] source slot (index among pixels with multiplicity > 0, or -1)
+//   [4] tap_lo [5] tap_hi  (range in the sorted tap table)  [6..7] unused
+constexpr int SPX_W = 8;
+// Sorted taps: (x, y, kind, column) -- column = output index in y[b, n, :].
+constexpr int TAP_W = 4;
+// Tile descriptor: one row of 16 int32 (domain coordinates):
+//   [0] R0 [1] RH      region rows  [R0, R0+RH)   (output rows + halo, clipped)
+//   [2] C0 [3] RW      region cols  [C0, C0+RW)
+//   [4] o_r0 [5] o_r1  output rows  [o_r0, o_r1)
+//   [6] o_c0 [7] o_c1  output cols  [o_c0, o_c1)
+//   [8] sp_lo [9] sp_hi  special pixels with x in region rows
+//   [10] tp_lo [11] tp_hi taps with x in output rows
+//   [12..15] unused
+constexpr int TILE_W = 16;
+
+// Arguments of the fused (temporally blocked) forward kernel.
+struct FwdArgs {
+  int B, Nx, Ny, T;
+  int ntiles;           // tiles per sample
+  const int32_t *tiles; // (ntiles, TILE_W)
+  const int32_t *spx;   // (nspx, SPX_W)
+  const int32_t *taps;  // (ntap, TAP_W) sorted
+  int ntap;
+  // linear coefficients (Nx,Ny) or, nonlinear, the base fields
+  const float *A, *Bm, *C2;                // linear
+  const float *bgeom, *clin, *rho, *rhoc;  // nonlinear: rhoc = rho*c_nl
+  float s2, dtm1, dtm2, t1, t4;
+  float sat_b0, sat_uth, cmin, cmax;
+  int nlf;  // nonlinear flags (bit0 saturable damping, bit1 Kerr, bit2 clamp)
+  // state in/out (B,Nx,Ny): u_n, u_{n-1}
+  const float *in1, *in2;
+  float *out1, *out2;
+  int n0, nsteps;
+  const float *x;  // (B,T)
+  float *y;        // (B,T,ntap) or null
+  float *yraw;     // (B,T,ntap) raw amplitudes for the adjoint, or null
+  float *fields;   // (B,T/fstride,Nx,Ny) or null
+  int fstride;
+  // record mode: lap(u_n) (and u_n for the nonlinear cell) of the steps of
+  // this launch into the segment buffer, domain-major with row stride Ny+1:
+  // rec[(b*K + (n - rec_n0)) * Nx*(Ny+1) + r*(Ny+1) + c]
+  float *rec_lap, *rec_u;
+  int rec_n0, rec_K;
+};
+
+// Arguments of the fused adjoint kernel (row strips only: C0 = 0, RW = Ny).
+struct AdjArgs {
+  int B, Nx, Ny, T;
+  int ntiles;
+  const int32_t *tiles;
+  const int32_t *spx;
+  const int32_t *taps;
+  int ntap;
+  int nsrcslot;
+  const float *A, *Bm, *C2;
+  const float *bgeom, *clin, *rho, *rhoc;
+  float s2, dtm1, dtm2, t1, t4;
+  float sat_b0, sat_uth, cmin, cmax;
+  int nlf;
+  // adjoint state: lam_{m+1}, lam_{m+2} in; lam_{m_lo}, lam_{m_lo+1} out
+  const float *in1, *in2;
+  float *out1, *out2;
+  int m_hi, nsteps;  // this launch forms lambda_m for m = m_hi .. m_hi-nsteps+1
+  const float *ybar; // (B,T,ntap) or null
+  const float *yraw; // (B,T,ntap)
+  const float *seg_lap, *seg_u;  // segment buffer written by the record pass
+  int seg_n0, seg_K;             // seg holds steps n = seg_n0 .. seg_n0+K-1
+  float *gacc;       // (B,Nx,Ny) running sum of lap(u_{m-1})*lambda_m (linear)
+  float *gacc_b, *gacc_rho;  // nonlinear accumulators (B,Nx,Ny)
+  float *gsrc;       // (B,T,nsrcslot) lambda_m at source pixels
+};
+
+// XCD-aware work-item order.  Workgroups are dispatched round-robin over the
+// 8 XCDs of the chip (workgroup i runs on XCD i % 8); remap so that
+// consecutive work items (the strips of one sample, which exchange halos
+// through L2 between launches) land on the same XCD and share its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  if ((nb & 7) != 0) return bid;
+  int per = nb >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+
+}  // namespace wt
+
+// Host-side helpers implemented in wt_runtime.hip.
+namespace wt {
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+}  // namespace wt
+
+#define WT_HIP(call)                                   \
+  do {                                                 \
+    hipError_t _e = (call);                            \
+    if (_e != hipSuccess) return wt::hip_fail(_e, #call); \
+  } while (0)

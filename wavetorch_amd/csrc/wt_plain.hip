@@ -1,0 +1,425 @@
+// wt_plain.hip -- one-launch-per-step kernels for gfx950.
+//
+//  * k_coef_linear      leapfrog coefficients (cell.py:13-15) once per forward
+//  * k_step_fwd         TimeStep.forward / _time_step (cell.py:12-24)
+//  * k_step_bwd         TimeStep.backward (cell.py:26-44)
+//  * k_loop_fwd_plain   one step of WaveRNN.forward (rnn.py:53-64): leapfrog +
+//                       source injection (source.py:15-22) + probes
+//                       (probe.py:14-27), optional record of lap(u_n) for the
+//                       adjoint's segment recompute
+//  * k_loop_adj_plain   one step of the discrete adjoint (cell.py:26-44 chained
+//                       by autograd over rnn.py:50-70)
+//
+// One thread per cell, fields read from HBM/L2 every step.  This is the
+// plain GPU form the temporally blocked kernels in wt_tiled.hip are measured
+// against, and the path for shapes the blocked kernels do not take (dense
+// field seeds, per-sample material fields).
+//
+// Every forward operation is written in the reference's order and compiled
+// without contraction (-ffp-contract=off), so the fp32 result equals the
+// reference's fp32 CPU result bit for bit when h == 1.
+#include "wt_internal.h"
+
+namespace wt {
+
+// 5-point Laplacian with zero padding, in conv2d's accumulation order
+// (operators.py:5-11): (((t1*up + t1*lf) + t4*ctr) + t1*rt) + t1*dn.
+template <bool UNIT_H>
+__device__ __forceinline__ float lap5(float up, float lf, float ct, float rt,
+                                      float dn, float t1, float t4) {
+  if (UNIT_H) {
+    float s = up + lf;
+    s = s + (-4.0f) * ct;
+    s = s + rt;
+    return s + dn;
+  } else {
+    float s = t1 * up;
+    s = s + t1 * lf;
+    s = s + t4 * ct;
+    s = s + t1 * rt;
+    return s + t1 * dn;
+  }
+}
+
+__global__ void k_coef_linear(const float *__restrict__ b,
+                              const float *__restrict__ c, int64_t n,
+                              float dtm1, float dtm2, float *__restrict__ A,
+                              float *__restrict__ Bm, float *__restrict__ C2) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float bd = b[i] * dtm1;
+  A[i] = 1.0f / (dtm2 + bd);
+  Bm[i] = dtm2 - bd;
+  float cc = c[i];
+  C2[i] = cc * cc;
+}
+
+// ------------------------------------------------------------- TimeStep
+
+template <bool UNIT_H>
+__global__ void k_step_fwd(const float *__restrict__ b,
+                           const float *__restrict__ c, int bc_ps,
+                           const float *__restrict__ y1,
+                           const float *__restrict__ y2, int B, int Nx, int Ny,
+                           float dtm1, float dtm2, float s2, float t1, float t4,
+                           float *__restrict__ y) {
+  int64_t F = (int64_t)Nx * Ny;
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= F * B) return;
+  int64_t q = gid % F;
+  int i = (int)(q / Ny), j = (int)(q % Ny);
+  const float *u = y1 + (gid - q);
+  float ct = u[q];
+  float up = i > 0 ? u[q - Ny] : 0.0f;
+  float dn = i < Nx - 1 ? u[q + Ny] : 0.0f;
+  float lf = j > 0 ? u[q - 1] : 0.0f;
+  float rt = j < Ny - 1 ? u[q + 1] : 0.0f;
+  float lap = lap5<UNIT_H>(up, lf, ct, rt, dn, t1, t4);
+  int64_t m = bc_ps ? gid : q;
+  float bd = b[m] * dtm1;
+  float A = 1.0f / (dtm2 + bd);
+  float Bm = dtm2 - bd;
+  float cc = c[m];
+  float c2 = cc * cc;
+  float p1 = s2 * ct;
+  float p2 = Bm * y2[gid];
+  float p3 = c2 * lap;
+  y[gid] = A * ((p1 - p2) + p3);
+}
+
+// TimeStep.backward (cell.py:26-44).  grad_b / grad_c of shared (Nx,Ny)
+// material fields are summed over the batch in-thread (deterministic).
+template <bool UNIT_H>
+__global__ void k_step_bwd(const float *__restrict__ b,
+                           const float *__restrict__ c, int bc_ps,
+                           const float *__restrict__ y1,
+                           const float *__restrict__ y2,
+                           const float *__restrict__ g, int B, int Nx, int Ny,
+                           float dt, float t1, float t4, float *__restrict__ gb,
+                           float *__restrict__ gc, float *__restrict__ gy1,
+                           float *__restrict__ gy2) {
+  int64_t F = (int64_t)Nx * Ny;
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // shared material: one thread per cell loops over the batch
+  int64_t total = bc_ps ? F * B : F;
+  if (gid >= total) return;
+  int nb = bc_ps ? 1 : B;
+  int64_t q = gid % F;
+  int i = (int)(q / Ny), j = (int)(q % Ny);
+  float dt2 = dt * dt;
+  float accb = 0.0f, accc = 0.0f;
+  for (int k = 0; k < nb; ++k) {
+    int64_t e = bc_ps ? gid : (int64_t)k * F + q;  // field element
+    int64_t base = e - q;
+    int64_t mq = bc_ps ? e : q;                     // material element
+    float bb = b[mq], cc = c[mq], gg = g[e];
+    float inv = 1.0f / (bb * dt + 1.0f);
+    if (gb || gc) {
+      const float *u = y1 + base;
+      float up = i > 0 ? u[q - Ny] : 0.0f;
+      float dn = i < Nx - 1 ? u[q + Ny] : 0.0f;
+      float lf = j > 0 ? u[q - 1] : 0.0f;
+      float rt = j < Ny - 1 ? u[q + 1] : 0.0f;
+      float L = lap5<UNIT_H>(up, lf, u[q], rt, dn, t1, t4);
+      if (gb) {
+        float t = cc * cc * dt2 * L + 2.0f * u[q] - 2.0f * y2[e];
+        float v = -(inv * inv) * dt * t * gg;
+        if (bc_ps) gb[e] = v; else accb += v;
+      }
+      if (gc) {
+        float v = inv * (2.0f * cc * dt2 * L) * gg;
+        if (bc_ps) gc[e] = v; else accc += v;
+      }
+    }
+    if (gy1) {
+      // dt^2 * L(c^2 g / (1+b dt)) + 2 g / (1+b dt)
+      auto cg = [&](int64_t qq) -> float {
+        int64_t mm = bc_ps ? base + qq : qq;
+        float b2 = b[mm], c2 = c[mm];
+        return (1.0f / (b2 * dt + 1.0f)) * (c2 * c2) * g[base + qq];
+      };
+      float up = i > 0 ? cg(q - Ny) : 0.0f;
+      float dn = i < Nx - 1 ? cg(q + Ny) : 0.0f;
+      float lf = j > 0 ? cg(q - 1) : 0.0f;
+      float rt = j < Ny - 1 ? cg(q + 1) : 0.0f;
+      float L = lap5<UNIT_H>(up, lf, cg(q), rt, dn, t1, t4);
+      gy1[e] = dt2 * L + 2.0f * gg * inv;
+    }
+    if (gy2) gy2[e] = (bb * dt - 1.0f) * inv * gg;
+  }
+  if (!bc_ps) {
+    if (gb) gb[q] = accb;
+    if (gc) gc[q] = accc;
+  }
+}
+
+// ------------------------------------------------------------- plain loop
+
+// Material of one cell for step n (cell.py:94-102 when NL != 0).
+template <int NL>
+__device__ __forceinline__ void cell_coef(const FwdArgs &a, int64_t q, float u,
+                                          float &A, float &Bm, float &c2) {
+  if (NL == 0) {
+    A = a.A[q];
+    Bm = a.Bm[q];
+    c2 = a.C2[q];
+  } else {
+    float bb = a.bgeom[q], cc = a.clin[q];
+    if (NL & 1) {
+      float r = u / a.sat_uth;
+      r = fabsf(r);
+      r = r * r;
+      r = r + 1.0f;
+      r = a.sat_b0 / r;
+      bb = bb + a.rho[q] * r;
+    }
+    if (NL & 2) cc = cc + a.rhoc[q] * (u * u);
+    if (NL & 4) cc = fminf(fmaxf(cc, a.cmin), a.cmax);
+    float bd = bb * a.dtm1;
+    A = 1.0f / (a.dtm2 + bd);
+    Bm = a.dtm2 - bd;
+    c2 = cc * cc;
+  }
+}
+
+// One forward step over all B*Nx*Ny cells: u_{n+1} = S(u_n, u_{n-1}) + sources.
+// spmap[q] = row of the special-pixel table or -1.
+template <bool UNIT_H, int NL, bool REC>
+__global__ void k_loop_fwd_plain(FwdArgs a, const int32_t *__restrict__ spmap,
+                                 int n) {
+  const int Nx = a.Nx, Ny = a.Ny;
+  const int64_t F = (int64_t)Nx * Ny;
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= F * a.B) return;
+  int b = (int)(gid / F);
+  int64_t q = gid - (int64_t)b * F;
+  int i = (int)(q / Ny), j = (int)(q - (int64_t)i * Ny);
+  const float *u = a.in1 + (int64_t)b * F;
+  float ct = u[q];
+  float up = i > 0 ? u[q - Ny] : 0.0f;
+  float dn = i < Nx - 1 ? u[q + Ny] : 0.0f;
+  float lf = j > 0 ? u[q - 1] : 0.0f;
+  float rt = j < Ny - 1 ? u[q + 1] : 0.0f;
+  float lap = lap5<UNIT_H>(up, lf, ct, rt, dn, a.t1, a.t4);
+  float A, Bm, c2;
+  cell_coef<NL>(a, q, ct, A, Bm, c2);
+  float p1 = a.s2 * ct;
+  float p2 = Bm * a.in2[gid];
+  float p3 = c2 * lap;
+  float v = A * ((p1 - p2) + p3);
+  int sp = spmap[q];
+  if (sp >= 0) {
+    const int32_t *e = a.spx + sp * SPX_W;
+    float xv = a.x[(int64_t)b * a.T + n];
+    for (int k = 0; k < e[2]; ++k) v = v + xv;
+    for (int t = e[4]; t < e[5]; ++t) {
+      const int32_t *tp = a.taps + t * TAP_W;
+      int64_t o = ((int64_t)b * a.T + n) * a.ntap + tp[3];
+      if (a.y) a.y[o] = tp[2] ? v * v : v;
+      if (a.yraw) a.yraw[o] = v;
+    }
+  }
+  a.out1[gid] = v;
+  if (a.fields && ((n + 1) % a.fstride) == 0) {
+    int64_t nf = a.T / a.fstride;
+    a.fields[((int64_t)b * nf + (n + 1) / a.fstride - 1) * F + q] = v;
+  }
+  if (REC) {
+    int64_t LWg = Ny + 1;
+    int64_t o = ((int64_t)b * a.rec_K + (n - a.rec_n0)) * Nx * LWg + i * LWg + j;
+    a.rec_lap[o] = lap;
+    if (NL) a.rec_u[o] = ct;
+  }
+}
+
+// One adjoint step (linear cell): forms lambda_m (m = n+1 indexes u_m):
+//   lambda_m = g_m + a1 lambda_{m+1} + L(w lambda_{m+1}) - a2 lambda_{m+2}
+// with a1 = s2*A, a2 = A*Bm, w = A*C2 (the transpose of cell.py:12-17; the
+// same recurrence autograd produces from cell.py:37-42 chained over steps),
+// then accumulates lap(u_{m-1}) * lambda_m for dJ/dc and records lambda_m at
+// source pixels for dJ/dx.  wl1 = w * lambda_{m+1} (written by the previous
+// step), wl_out = w * lambda_m.
+template <bool UNIT_H>
+__global__ void k_loop_adj_plain(AdjArgs a, const int32_t *__restrict__ spmap,
+                                 const float *__restrict__ wl1,
+                                 float *__restrict__ wl_out,
+                                 const float *__restrict__ fbar, int m) {
+  const int Nx = a.Nx, Ny = a.Ny;
+  const int64_t F = (int64_t)Nx * Ny;
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= F * a.B) return;
+  int b = (int)(gid / F);
+  int64_t q = gid - (int64_t)b * F;
+  int i = (int)(q / Ny), j = (int)(q - (int64_t)i * Ny);
+  const float *W = wl1 + (int64_t)b * F;
+  float up = i > 0 ? W[q - Ny] : 0.0f;
+  float dn = i < Nx - 1 ? W[q + Ny] : 0.0f;
+  float lf = j > 0 ? W[q - 1] : 0.0f;
+  float rt = j < Ny - 1 ? W[q + 1] : 0.0f;
+  float lapw = lap5<UNIT_H>(up, lf, W[q], rt, dn, a.t1, a.t4);
+  float A = a.A[q];
+  float a1 = A * a.s2, a2 = A * a.Bm[q], w = A * a.C2[q];
+  float lam = fmaf(-a2, a.in2[gid], fmaf(a1, a.in1[gid], lapw));
+  int sp = spmap[q];
+  if (sp >= 0) {
+    const int32_t *e = a.spx + sp * SPX_W;
+    if (a.ybar) {
+      for (int t = e[4]; t < e[5]; ++t) {
+        const int32_t *tp = a.taps + t * TAP_W;
+        int64_t o = ((int64_t)b * a.T + (m - 1)) * a.ntap + tp[3];
+        float gy = a.ybar[o];
+        lam += tp[2] ? 2.0f * a.yraw[o] * gy : gy;
+      }
+    }
+  }
+  if (fbar) lam += fbar[((int64_t)b * a.T + (m - 1)) * F + q];
+  a.out1[gid] = lam;
+  wl_out[gid] = w * lam;
+  int64_t LWg = Ny + 1;
+  int64_t o = ((int64_t)b * a.seg_K + (m - 1 - a.seg_n0)) * Nx * LWg + i * LWg + j;
+  a.gacc[gid] = fmaf(a.seg_lap[o], lam, a.gacc[gid]);
+  if (sp >= 0) {
+    const int32_t *e = a.spx + sp * SPX_W;
+    if (e[2] > 0)
+      a.gsrc[((int64_t)b * a.T + (m - 1)) * a.nsrcslot + e[3]] = lam;
+  }
+}
+
+// dJ/dc = 2 c A sum_b gacc_b (the batch reduction autograd performs on
+// grad_c, cell.py:36); deterministic in-thread order.
+__global__ void k_finalize_gc(const float *__restrict__ gacc, int B, int64_t F,
+                              const float *__restrict__ c,
+                              const float *__restrict__ A,
+                              float *__restrict__ gc) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= F) return;
+  float s = 0.0f;
+  for (int b = 0; b < B; ++b) s += gacc[(int64_t)b * F + q];
+  gc[q] = 2.0f * c[q] * A[q] * s;
+}
+
+// dJ/dx[b, n] = sum over source pixels of multiplicity * lambda_{n+1}[pixel].
+__global__ void k_finalize_gx(const float *__restrict__ gsrc, int B, int T,
+                              int nslot, const int32_t *__restrict__ mult,
+                              float *__restrict__ gx) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)B * T) return;
+  float s = 0.0f;
+  for (int k = 0; k < nslot; ++k) s += (float)mult[k] * gsrc[i * nslot + k];
+  gx[i] = s;
+}
+
+// Fill kernel used instead of hipMemsetAsync so that captured graphs carry
+// no memset nodes.
+__global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// ------------------------------------------------------------- launchers
+
+static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_coef_linear(const float *b, const float *c, int64_t n, float dtm1,
+                              float dtm2, float *A, float *Bm, float *C2, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_coef_linear, dim3(nblk(n, 256)), dim3(256), 0, s, b, c, n, dtm1, dtm2,
+                     A, Bm, C2);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_fwd(bool unit_h, const float *b, const float *c, int bc_ps,
+                           const float *y1, const float *y2, int B, int Nx, int Ny, float dtm1,
+                           float dtm2, float s2, float t1, float t4, float *y, hipStream_t s) {
+  int64_t n = (int64_t)B * Nx * Ny;
+  if (n <= 0) return hipSuccess;
+  if (unit_h)
+    hipLaunchKernelGGL(k_step_fwd<true>, dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps, y1,
+                       y2, B, Nx, Ny, dtm1, dtm2, s2, t1, t4, y);
+  else
+    hipLaunchKernelGGL(k_step_fwd<false>, dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps, y1,
+                       y2, B, Nx, Ny, dtm1, dtm2, s2, t1, t4, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_bwd(bool unit_h, const float *b, const float *c, int bc_ps,
+                           const float *y1, const float *y2, const float *g, int B, int Nx,
+                           int Ny, float dt, float t1, float t4, float *gb, float *gc,
+                           float *gy1, float *gy2, hipStream_t s) {
+  int64_t n = bc_ps ? (int64_t)B * Nx * Ny : (int64_t)Nx * Ny;
+  if (n <= 0) return hipSuccess;
+  // with shared material one thread walks the batch of a cell; the
+  // per-sample outputs grad_y1/grad_y2 are written inside that walk.
+  if (unit_h)
+    hipLaunchKernelGGL(k_step_bwd<true>, dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps, y1,
+                       y2, g, B, Nx, Ny, dt, t1, t4, gb, gc, gy1, gy2);
+  else
+    hipLaunchKernelGGL(k_step_bwd<false>, dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps, y1,
+                       y2, g, B, Nx, Ny, dt, t1, t4, gb, gc, gy1, gy2);
+  return hipGetLastError();
+}
+
+hipError_t launch_fwd_plain(bool unit_h, int nl, bool rec, const FwdArgs &a,
+                            const int32_t *spmap, int n, hipStream_t s) {
+  int64_t tot = (int64_t)a.B * a.Nx * a.Ny;
+  dim3 g(nblk(tot, 256)), bsz(256);
+#define FP(UH, NL, REC) \
+  hipLaunchKernelGGL((k_loop_fwd_plain<UH, NL, REC>), g, bsz, 0, s, a, spmap, n)
+#define FP_NL(UH, REC)                     \
+  switch (nl) {                            \
+    case 0: FP(UH, 0, REC); break;         \
+    case 1: FP(UH, 1, REC); break;         \
+    case 2: FP(UH, 2, REC); break;         \
+    case 3: FP(UH, 3, REC); break;         \
+    case 5: FP(UH, 5, REC); break;         \
+    case 6: FP(UH, 6, REC); break;         \
+    case 7: FP(UH, 7, REC); break;         \
+    default: return hipErrorInvalidValue;  \
+  }
+  if (unit_h) {
+    if (rec) { FP_NL(true, true) } else { FP_NL(true, false) }
+  } else {
+    if (rec) { FP_NL(false, true) } else { FP_NL(false, false) }
+  }
+#undef FP_NL
+#undef FP
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_plain(bool unit_h, const AdjArgs &a, const int32_t *spmap,
+                            const float *wl1, float *wl_out, const float *fbar, int m,
+                            hipStream_t s) {
+  int64_t tot = (int64_t)a.B * a.Nx * a.Ny;
+  if (unit_h)
+    hipLaunchKernelGGL(k_loop_adj_plain<true>, dim3(nblk(tot, 256)), dim3(256), 0, s, a, spmap,
+                       wl1, wl_out, fbar, m);
+  else
+    hipLaunchKernelGGL(k_loop_adj_plain<false>, dim3(nblk(tot, 256)), dim3(256), 0, s, a, spmap,
+                       wl1, wl_out, fbar, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_gc(const float *gacc, int B, int64_t F, const float *c,
+                              const float *A, float *gc, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize_gc, dim3(nblk(F, 256)), dim3(256), 0, s, gacc, B, F, c, A, gc);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_gx(const float *gsrc, int B, int T, int nslot, const int32_t *mult,
+                              float *gx, hipStream_t s) {
+  int64_t n = (int64_t)B * T;
+  hipLaunchKernelGGL(k_finalize_gx, dim3(nblk(n, 256)), dim3(256), 0, s, gsrc, B, T, nslot,
+                     mult, gx);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = nblk(n, 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace wt

@@ -1,0 +1,839 @@
+// wt_runtime.hip -- host runtime and C ABI of libwavetorch_hip.so.
+//
+// Plans (validated configuration + device tables + tiling), workspace
+// layout, the forward time loop with checkpoints every K steps, the reverse
+// sweep that recomputes each K-step segment from its checkpoint and runs the
+// adjoint through it, the single-step TimeStep entry points and HIP graph
+// capture.  See include/wavetorch_hip.h for the contract of every export.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "wt_internal.h"
+
+namespace wt {
+
+// ---------------------------------------------------------------- errors
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  set_error("HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+  return WT_EHIP;
+}
+
+// launchers implemented in wt_plain.hip / wt_tiled.hip / wt_geom.hip
+hipError_t launch_coef_linear(const float *b, const float *c, int64_t n, float dtm1, float dtm2,
+                              float *A, float *Bm, float *C2, hipStream_t s);
+hipError_t launch_step_fwd(bool unit_h, const float *b, const float *c, int bc_ps,
+                           const float *y1, const float *y2, int B, int Nx, int Ny, float dtm1,
+                           float dtm2, float s2, float t1, float t4, float *y, hipStream_t s);
+hipError_t launch_step_bwd(bool unit_h, const float *b, const float *c, int bc_ps,
+                           const float *y1, const float *y2, const float *g, int B, int Nx,
+                           int Ny, float dt, float t1, float t4, float *gb, float *gc,
+                           float *gy1, float *gy2, hipStream_t s);
+hipError_t launch_fwd_plain(bool unit_h, int nl, bool rec, const FwdArgs &a,
+                            const int32_t *spmap, int n, hipStream_t s);
+hipError_t launch_adj_plain(bool unit_h, const AdjArgs &a, const int32_t *spmap,
+                            const float *wl1, float *wl_out, const float *fbar, int m,
+                            hipStream_t s);
+hipError_t launch_finalize_gc(const float *gacc, int B, int64_t F, const float *c,
+                              const float *A, float *gc, hipStream_t s);
+hipError_t launch_finalize_gx(const float *gsrc, int B, int T, int nslot, const int32_t *mult,
+                              float *gx, hipStream_t s);
+hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t s);
+hipError_t launch_fwd_tiled(int nt, int cpt, bool unit_h, int nl, bool rec, const FwdArgs &a,
+                            int LWmax, hipStream_t s);
+hipError_t launch_adj_tiled(int nt, int cpt, bool unit_h, const AdjArgs &a, int LWmax,
+                            hipStream_t s);
+bool tiled_variant_exists(int nt, int cpt);
+size_t tiled_shmem_bytes(int nt, int cpt, int LW);
+hipError_t launch_mul_scalar(const float *in, float v, int64_t n, float *out, hipStream_t s);
+hipError_t launch_clamp(const float *in, float lo, float hi, int64_t n, float *out,
+                        hipStream_t s);
+
+constexpr int SRC_SLOTS_HOST = 4;  // must match wt_tiled.hip
+constexpr int SPC_SLOTS_HOST = 2;
+constexpr size_t LDS_LIMIT = 160 * 1024;
+
+// ---------------------------------------------------------------- plan
+
+struct Plan {
+  int dev = 0;
+  int B = 0, T = 0, Nx = 0, Ny = 0;
+  float dt = 0, h = 1;
+  bool unit_h = true;
+  int nl = 0;  // 0 linear; bit0 saturable damping, bit1 Kerr speed, bit2 clamp
+  bool clamp_only = false;
+  float sat_b0 = 0, sat_uth = 1, c_nl = 0, cmin = 0, cmax = 0;
+  float dtm1 = 0, dtm2 = 0, s2 = 0, t1 = 1, t4 = -4;
+  wt_tuning tun{};
+  int K = 64, nseg = 1;
+  int nspx = 0, ntap = 0, nslot = 0;
+  std::vector<int32_t> h_spx, h_taps, h_tiles, h_mult;
+  int32_t *d_spx = nullptr, *d_taps = nullptr, *d_spmap = nullptr, *d_mult = nullptr;
+  int32_t *d_tiles = nullptr;
+  int ntiles = 1, LWmax = 0;
+  bool tiled = false;
+  // workspace layout (bytes)
+  size_t F = 0;
+  size_t saved_coef = 0, saved_ck = 0, saved_yraw = 0, saved_bytes = 0;
+  size_t t_state = 0, t_lam = 0, t_wl = 0, t_gacc = 0, t_seg = 0, t_gsrc = 0, t_extra = 0,
+         temp_bytes = 0;
+  size_t temp_bytes_nograd = 0;
+};
+
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static void free_plan(Plan *p) {
+  if (!p) return;
+  if (p->d_spx) (void)hipFree(p->d_spx);
+  if (p->d_taps) (void)hipFree(p->d_taps);
+  if (p->d_spmap) (void)hipFree(p->d_spmap);
+  if (p->d_mult) (void)hipFree(p->d_mult);
+  if (p->d_tiles) (void)hipFree(p->d_tiles);
+  delete p;
+}
+
+// Row strips (optionally 2-D tiles when `cols` > 1) with halo `halo`.
+static void build_tiles(Plan &p, int strips, int cols, int halo) {
+  p.h_tiles.clear();
+  p.LWmax = 0;
+  for (int si = 0; si < strips; ++si) {
+    int o_r0 = (int)((int64_t)p.Nx * si / strips), o_r1 = (int)((int64_t)p.Nx * (si + 1) / strips);
+    for (int ci = 0; ci < cols; ++ci) {
+      int o_c0 = (int)((int64_t)p.Ny * ci / cols), o_c1 = (int)((int64_t)p.Ny * (ci + 1) / cols);
+      int R0 = std::max(0, o_r0 - (strips > 1 ? halo : 0));
+      int R1 = std::min(p.Nx, o_r1 + (strips > 1 ? halo : 0));
+      int C0 = std::max(0, o_c0 - (cols > 1 ? halo : 0));
+      int C1 = std::min(p.Ny, o_c1 + (cols > 1 ? halo : 0));
+      int32_t t[TILE_W] = {0};
+      t[0] = R0; t[1] = R1 - R0; t[2] = C0; t[3] = C1 - C0;
+      t[4] = o_r0; t[5] = o_r1; t[6] = o_c0; t[7] = o_c1;
+      // special pixels sorted by x: range with x in [R0, R1)
+      int lo = 0, hi = 0;
+      while (lo < p.nspx && p.h_spx[lo * SPX_W] < R0) ++lo;
+      hi = lo;
+      while (hi < p.nspx && p.h_spx[hi * SPX_W] < R1) ++hi;
+      t[8] = lo; t[9] = hi;
+      int tl = 0, th = 0;
+      while (tl < p.ntap && p.h_taps[tl * TAP_W] < o_r0) ++tl;
+      th = tl;
+      while (th < p.ntap && p.h_taps[th * TAP_W] < o_r1) ++th;
+      t[10] = tl; t[11] = th;
+      p.h_tiles.insert(p.h_tiles.end(), t, t + TILE_W);
+      p.LWmax = std::max(p.LWmax, (C1 - C0) + 1);
+    }
+  }
+  p.ntiles = strips * cols;
+}
+
+// Checks that the tiled kernels can take this tiling: region fits the
+// (nt, cpt) variant and LDS, per-thread source/special/tap slots suffice.
+static bool tiles_ok(const Plan &p, int nt, int cpt) {
+  if (!tiled_variant_exists(nt, cpt)) return false;
+  if (tiled_shmem_bytes(nt, cpt, p.LWmax) > LDS_LIMIT) return false;
+  for (int ti = 0; ti < p.ntiles; ++ti) {
+    const int32_t *t = &p.h_tiles[ti * TILE_W];
+    int RH = t[1], RW = t[3], LW = RW + 1;
+    if ((int64_t)RH * LW > (int64_t)nt * cpt) return false;
+    std::vector<int> srcn(nt, 0), spcn(nt, 0);
+    for (int k = t[8]; k < t[9]; ++k) {
+      const int32_t *e = &p.h_spx[k * SPX_W];
+      int rr = e[0] - t[0], yy = e[1] - t[2];
+      if (rr < 0 || rr >= RH || yy < 0 || yy >= RW) continue;
+      int f = rr * LW + yy;
+      srcn[f % nt] += e[2];
+      spcn[f % nt] += 1;
+    }
+    for (int i = 0; i < nt; ++i)
+      if (srcn[i] > SRC_SLOTS_HOST || spcn[i] > SPC_SLOTS_HOST) return false;
+    int ntp = 0;
+    for (int k = t[10]; k < t[11]; ++k) {
+      const int32_t *tp = &p.h_taps[k * TAP_W];
+      if (tp[1] >= t[6] && tp[1] < t[7]) ++ntp;
+    }
+    if (ntp > nt) return false;
+  }
+  return true;
+}
+
+static const int kVariants[][2] = {{1024, 8}, {1024, 12}, {1024, 16}, {512, 16},
+                                   {512, 24}, {512, 32},  {256, 32},  {256, 48}};
+
+// Choose (nt, cpt) for the current tiling: the smallest region capacity
+// that fits, preferring more threads.
+static bool pick_variant(Plan &p, int &nt, int &cpt) {
+  int best = -1;
+  int64_t best_cap = 0;
+  for (int v = 0; v < (int)(sizeof(kVariants) / sizeof(kVariants[0])); ++v) {
+    int a = kVariants[v][0], c = kVariants[v][1];
+    if (!tiles_ok(p, a, c)) continue;
+    int64_t cap = (int64_t)a * c;
+    if (best < 0 || cap < best_cap || (cap == best_cap && a > kVariants[best][0])) {
+      best = v;
+      best_cap = cap;
+    }
+  }
+  if (best < 0) return false;
+  nt = kVariants[best][0];
+  cpt = kVariants[best][1];
+  return true;
+}
+
+static int plan_layout(Plan &p) {
+  size_t F = (size_t)p.Nx * p.Ny;
+  p.F = F;
+  size_t BF = (size_t)p.B * F * sizeof(float);
+  bool nlc = p.nl != 0;
+  // saved: coefficients (3F) or rho*c_nl (F), checkpoints, probe amplitudes
+  p.saved_coef = align_up((nlc ? 1 : 3) * F * sizeof(float));
+  p.saved_ck = align_up((size_t)p.nseg * 2 * BF);
+  p.saved_yraw = align_up((size_t)p.B * p.T * std::max(p.ntap, 1) * sizeof(float));
+  p.saved_bytes = p.saved_coef + p.saved_ck + p.saved_yraw;
+  // temp
+  p.t_state = align_up(6 * BF);  // tiled: 2x2 ping-pong; plain: 3 rotating (+ spare)
+  p.t_lam = align_up(4 * BF);    // adjoint state ping-pong (plain uses 3)
+  p.t_wl = align_up(2 * BF);     // plain adjoint w*lambda
+  p.t_gacc = align_up((nlc ? 3 : 1) * BF);
+  size_t segcells = (size_t)p.B * p.K * p.Nx * (p.Ny + 1);
+  p.t_seg = align_up((nlc ? 2 : 1) * segcells * sizeof(float));
+  p.t_gsrc = align_up((size_t)p.B * p.T * std::max(p.nslot, 1) * sizeof(float));
+  p.t_extra = align_up(3 * F * sizeof(float));  // coefficients when not saving
+  p.temp_bytes = p.t_state + p.t_lam + p.t_wl + p.t_gacc + p.t_seg + p.t_gsrc + p.t_extra;
+  p.temp_bytes_nograd = p.t_state + p.t_extra;
+  return WT_OK;
+}
+
+}  // namespace wt
+
+using namespace wt;
+
+// ====================================================================== ABI
+
+extern "C" {
+
+int wt_abi_version(void) { return WT_ABI_VERSION; }
+
+int wt_last_error(char *buf, size_t len) {
+  const std::string &s = g_last_error;
+  if (buf && len > 0) {
+    size_t n = std::min(len - 1, s.size());
+    memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  return (int)s.size();
+}
+
+static void scalars(float dt, float h, float &dtm1, float &dtm2, float &s2, float &t1,
+                    float &t4) {
+  // dt**-2 == 1/(dt*dt), dt**-1 == 1/dt, 2/dt**2 == 2*dt**-2 for the fp32
+  // 0-dim tensors of cell.py:13-14; h**(-2) == 1/(h*h) (operators.py:7).
+  dtm2 = 1.0f / (dt * dt);
+  dtm1 = 1.0f / dt;
+  s2 = 2.0f * dtm2;
+  t1 = 1.0f / (h * h);
+  t4 = -4.0f * t1;
+}
+
+int wt_coef_linear(const float *b, const float *c, int64_t n, float dt, float *A, float *Bm,
+                   float *C2, void *stream) {
+  if (!b || !c || !A || !Bm || !C2 || n < 0 || !(dt > 0)) {
+    set_error("wt_coef_linear: invalid argument");
+    return WT_EINVAL;
+  }
+  float dtm1, dtm2, s2, t1, t4;
+  scalars(dt, 1.0f, dtm1, dtm2, s2, t1, t4);
+  WT_HIP(launch_coef_linear(b, c, n, dtm1, dtm2, A, Bm, C2, (hipStream_t)stream));
+  return WT_OK;
+}
+
+int wt_step_fwd(const float *b, const float *c, int32_t bc_per_sample, const float *y1,
+                const float *y2, int32_t B, int32_t Nx, int32_t Ny, float dt, float h, float *y,
+                void *stream) {
+  if (!b || !c || !y1 || !y2 || !y || B < 0 || Nx <= 0 || Ny <= 0 || !(dt > 0) || !(h > 0)) {
+    set_error("wt_step_fwd: invalid argument");
+    return WT_EINVAL;
+  }
+  float dtm1, dtm2, s2, t1, t4;
+  scalars(dt, h, dtm1, dtm2, s2, t1, t4);
+  WT_HIP(launch_step_fwd(h == 1.0f, b, c, bc_per_sample, y1, y2, B, Nx, Ny, dtm1, dtm2, s2, t1,
+                         t4, y, (hipStream_t)stream));
+  return WT_OK;
+}
+
+int wt_step_bwd(const float *b, const float *c, int32_t bc_per_sample, const float *y1,
+                const float *y2, const float *g, int32_t B, int32_t Nx, int32_t Ny, float dt,
+                float h, float *grad_b, float *grad_c, float *grad_y1, float *grad_y2,
+                void *stream) {
+  if (!b || !c || !y1 || !y2 || !g || B < 0 || Nx <= 0 || Ny <= 0 || !(dt > 0) || !(h > 0)) {
+    set_error("wt_step_bwd: invalid argument");
+    return WT_EINVAL;
+  }
+  float dtm1, dtm2, s2, t1, t4;
+  scalars(dt, h, dtm1, dtm2, s2, t1, t4);
+  WT_HIP(launch_step_bwd(h == 1.0f, b, c, bc_per_sample, y1, y2, g, B, Nx, Ny, dt, t1, t4,
+                         grad_b, grad_c, grad_y1, grad_y2, (hipStream_t)stream));
+  return WT_OK;
+}
+
+int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
+                   const wt_tuning *tuning, void **plan_out) {
+  if (!cfg || !plan_out) {
+    set_error("wt_plan_create: null argument");
+    return WT_EINVAL;
+  }
+  *plan_out = nullptr;
+  if (B <= 0 || T <= 0 || cfg->Nx < 3 || cfg->Ny < 3 || !(cfg->dt > 0) || !(cfg->h > 0)) {
+    set_error("wt_plan_create: invalid shape B=%d T=%d Nx=%d Ny=%d dt=%g h=%g", B, T, cfg->Nx,
+              cfg->Ny, cfg->dt, cfg->h);
+    return WT_EINVAL;
+  }
+  if ((int64_t)cfg->Nx * (cfg->Ny + 1) >= (1 << 22)) {
+    set_error("wt_plan_create: grid %dx%d too large for int32 cell indexing", cfg->Nx, cfg->Ny);
+    return WT_EUNSUPPORTED;
+  }
+  // CFL check of cell.py:66-73: dt > h / (cmax * sqrt(2)) is rejected.
+  if (cmax_geom > 0 && (double)cfg->dt > 1.0 / cmax_geom * cfg->h / std::sqrt(2.0)) {
+    set_error("The spatial discretization defined by the geometry `h = %f` and the temporal "
+              "discretization defined by the model `dt = %f` do not satisfy the CFL stability "
+              "criteria",
+              cfg->h, cfg->dt);
+    return WT_ECFL;
+  }
+  if (cfg->nsrc_px < 0 || cfg->ntap < 0 || (cfg->nsrc_px > 0 && !cfg->src_px) ||
+      (cfg->ntap > 0 && !cfg->taps)) {
+    set_error("wt_plan_create: invalid source/probe tables");
+    return WT_EINVAL;
+  }
+  Plan *p = new Plan();
+  p->B = B;
+  p->T = T;
+  p->Nx = cfg->Nx;
+  p->Ny = cfg->Ny;
+  p->dt = cfg->dt;
+  p->h = cfg->h;
+  p->unit_h = cfg->h == 1.0f;
+  p->sat_b0 = cfg->sat_b0;
+  p->sat_uth = cfg->sat_uth;
+  p->c_nl = cfg->c_nl;
+  p->cmin = cfg->c_min;
+  p->cmax = cfg->c_max;
+  int nl = 0;
+  if (cfg->sat_b0 > 0) nl |= 1;
+  if (cfg->c_nl != 0) nl |= 2;
+  if (cfg->c_clamp) nl |= 4;
+  if (nl == 4) {  // clamp of a time-invariant c: applied once to c_linear
+    p->clamp_only = true;
+    nl = 0;
+  }
+  p->nl = nl;
+  scalars(p->dt, p->h, p->dtm1, p->dtm2, p->s2, p->t1, p->t4);
+  (void)hipGetDevice(&p->dev);
+
+  // ---- special pixels: sources (with multiplicity) and probe taps
+  struct Px { int x, y, mult, slot, tlo, thi; };
+  std::vector<Px> px;
+  auto find_or_add = [&](int x, int y) -> int {
+    for (size_t i = 0; i < px.size(); ++i)
+      if (px[i].x == x && px[i].y == y) return (int)i;
+    px.push_back(Px{x, y, 0, -1, 0, 0});
+    return (int)px.size() - 1;
+  };
+  for (int k = 0; k < cfg->nsrc_px; ++k) {
+    int x = cfg->src_px[3 * k], y = cfg->src_px[3 * k + 1];
+    if (x < 0 || x >= p->Nx || y < 0 || y >= p->Ny) {
+      set_error("source pixel (%d, %d) outside the %dx%d domain", x, y, p->Nx, p->Ny);
+      free_plan(p);
+      return WT_EINVAL;
+    }
+    px[find_or_add(x, y)].mult += 1;
+  }
+  struct Tp { int x, y, kind, col; };
+  std::vector<Tp> tp;
+  for (int k = 0; k < cfg->ntap; ++k) {
+    int x = cfg->taps[3 * k], y = cfg->taps[3 * k + 1], kind = cfg->taps[3 * k + 2];
+    if (x < 0 || x >= p->Nx || y < 0 || y >= p->Ny || (kind != 0 && kind != 1)) {
+      set_error("probe tap (%d, %d, kind %d) invalid for the %dx%d domain", x, y, kind, p->Nx,
+                p->Ny);
+      free_plan(p);
+      return WT_EINVAL;
+    }
+    find_or_add(x, y);
+    tp.push_back(Tp{x, y, kind, k});
+  }
+  std::sort(px.begin(), px.end(),
+            [](const Px &a, const Px &b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+  std::sort(tp.begin(), tp.end(), [](const Tp &a, const Tp &b) {
+    return a.x != b.x ? a.x < b.x : (a.y != b.y ? a.y < b.y : a.col < b.col);
+  });
+  p->nspx = (int)px.size();
+  p->ntap = (int)tp.size();
+  int nslot = 0;
+  for (auto &q : px) {
+    if (q.mult > 0) {
+      q.slot = nslot++;
+      p->h_mult.push_back(q.mult);
+    }
+    int lo = 0;
+    while (lo < (int)tp.size() && (tp[lo].x != q.x || tp[lo].y != q.y)) ++lo;
+    int hi = lo;
+    while (hi < (int)tp.size() && tp[hi].x == q.x && tp[hi].y == q.y) ++hi;
+    q.tlo = lo;
+    q.thi = hi;
+  }
+  p->nslot = nslot;
+  for (auto &q : px) {
+    int32_t r[SPX_W] = {q.x, q.y, q.mult, q.slot, q.tlo, q.thi, 0, 0};
+    p->h_spx.insert(p->h_spx.end(), r, r + SPX_W);
+  }
+  for (auto &t : tp) {
+    int32_t r[TAP_W] = {t.x, t.y, t.kind, t.col};
+    p->h_taps.insert(p->h_taps.end(), r, r + TAP_W);
+  }
+
+  // ---- tuning
+  wt_tuning tun{};
+  if (tuning) tun = *tuning;
+  p->K = tun.ck_every > 0 ? tun.ck_every : 64;
+  if (p->K > T) p->K = T;
+  p->nseg = (T + p->K - 1) / p->K;
+  bool tiled = !tun.plain && p->unit_h;
+  if (tiled) {
+    int strips = tun.strips, tb = tun.tblock, nt = tun.nt, cpt = tun.cpt;
+    if (tb <= 0) tb = 16;
+    if (tb > p->K) tb = p->K;
+    if (strips <= 0) {
+      // heuristic: enough workgroups to cover the 256 CUs, region fitting
+      // the largest variant
+      strips = 1;
+      while (strips < p->Nx / 4 && (int64_t)B * strips < 256) ++strips;
+    }
+    bool ok = false;
+    for (; strips <= p->Nx; ++strips) {
+      if (strips > 1 && tb > (p->Nx / strips)) {
+        // a halo deeper than the strip is allowed but wasteful; keep going
+      }
+      build_tiles(*p, strips, 1, tb);
+      if (nt > 0 && cpt > 0) {
+        ok = tiles_ok(*p, nt, cpt);
+        if (!ok && tun.strips > 0) break;
+      } else {
+        ok = pick_variant(*p, nt, cpt);
+      }
+      if (ok) break;
+      if (tun.strips > 0) break;
+    }
+    if (!ok) {
+      if (tun.strips > 0 || tun.nt > 0) {
+        set_error("wt_plan_create: tuning (strips=%d tblock=%d nt=%d cpt=%d) does not fit",
+                  tun.strips, tun.tblock, tun.nt, tun.cpt);
+        free_plan(p);
+        return WT_EUNSUPPORTED;
+      }
+      tiled = false;
+    } else {
+      tun.strips = strips;
+      tun.tblock = tb;
+      tun.nt = nt;
+      tun.cpt = cpt;
+    }
+  }
+  if (!tiled) {
+    tun.plain = 1;
+    tun.strips = 0;
+    tun.tblock = 1;
+    tun.nt = 256;
+    tun.cpt = 1;
+  }
+  tun.ck_every = p->K;
+  p->tun = tun;
+  p->tiled = tiled;
+  plan_layout(*p);
+
+  // ---- device tables
+  auto up = [&](int32_t **dst, const std::vector<int32_t> &v) -> int {
+    size_t n = std::max<size_t>(v.size(), 1) * sizeof(int32_t);
+    hipError_t e = hipMalloc((void **)dst, n);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(plan table)");
+    if (!v.empty()) {
+      e = hipMemcpy(*dst, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hip_fail(e, "hipMemcpy(plan table)");
+    }
+    return WT_OK;
+  };
+  std::vector<int32_t> spmap((size_t)p->Nx * p->Ny, -1);
+  for (int k = 0; k < p->nspx; ++k)
+    spmap[(size_t)p->h_spx[k * SPX_W] * p->Ny + p->h_spx[k * SPX_W + 1]] = k;
+  int rc;
+  if ((rc = up(&p->d_spx, p->h_spx)) || (rc = up(&p->d_taps, p->h_taps)) ||
+      (rc = up(&p->d_spmap, spmap)) || (rc = up(&p->d_mult, p->h_mult)) ||
+      (rc = up(&p->d_tiles, p->h_tiles))) {
+    free_plan(p);
+    return rc;
+  }
+  *plan_out = p;
+  return WT_OK;
+}
+
+int wt_plan_destroy(void *plan) {
+  free_plan((Plan *)plan);
+  return WT_OK;
+}
+
+int wt_plan_tuning(const void *plan, wt_tuning *out) {
+  if (!plan || !out) {
+    set_error("wt_plan_tuning: null argument");
+    return WT_EINVAL;
+  }
+  *out = ((const Plan *)plan)->tun;
+  return WT_OK;
+}
+
+int wt_workspace_bytes(const void *plan, int32_t with_grad, int64_t *saved, int64_t *temp) {
+  if (!plan) {
+    set_error("wt_workspace_bytes: null plan");
+    return WT_EINVAL;
+  }
+  const Plan *p = (const Plan *)plan;
+  if (saved) *saved = with_grad ? (int64_t)p->saved_bytes : 0;
+  if (temp) *temp = with_grad ? (int64_t)p->temp_bytes : (int64_t)p->temp_bytes_nograd;
+  return WT_OK;
+}
+
+// ---------------------------------------------------------------- loops
+
+namespace {
+
+struct Ws {
+  float *A, *Bm, *C2, *rhoc;  // coefficients
+  float *ck;                  // checkpoints (nseg, 2, B, F)
+  float *yraw;
+  float *st;                  // 6 state fields
+  float *lam;                 // 4 adjoint fields
+  float *wl;                  // 2 fields
+  float *gacc;                // 1 (linear) or 3 fields
+  float *seg;                 // segment record
+  float *gsrc;
+};
+
+Ws layout(const Plan &p, void *saved, void *temp, bool with_grad) {
+  Ws w{};
+  char *t = (char *)temp;
+  w.st = (float *)t;
+  t += p.t_state;
+  float *coef;
+  if (with_grad) {
+    char *s = (char *)saved;
+    coef = (float *)s;
+    s += p.saved_coef;
+    w.ck = (float *)s;
+    s += p.saved_ck;
+    w.yraw = (float *)s;
+    w.lam = (float *)t;
+    t += p.t_lam;
+    w.wl = (float *)t;
+    t += p.t_wl;
+    w.gacc = (float *)t;
+    t += p.t_gacc;
+    w.seg = (float *)t;
+    t += p.t_seg;
+    w.gsrc = (float *)t;
+    t += p.t_gsrc;
+  } else {
+    coef = (float *)t;
+  }
+  if (p.nl) {
+    w.rhoc = coef;
+    w.A = w.Bm = w.C2 = nullptr;
+  } else {
+    w.A = coef;
+    w.Bm = coef + p.F;
+    w.C2 = coef + 2 * p.F;
+    w.rhoc = nullptr;
+  }
+  return w;
+}
+
+FwdArgs base_fwd(const Plan &p, const Ws &w, const float *b, const float *c, const float *rho,
+                 const float *x) {
+  FwdArgs a{};
+  a.B = p.B; a.Nx = p.Nx; a.Ny = p.Ny; a.T = p.T;
+  a.ntiles = p.ntiles; a.tiles = p.d_tiles; a.spx = p.d_spx; a.taps = p.d_taps;
+  a.ntap = p.ntap;
+  a.A = w.A; a.Bm = w.Bm; a.C2 = w.C2;
+  a.bgeom = b; a.clin = c; a.rho = rho; a.rhoc = w.rhoc;
+  a.s2 = p.s2; a.dtm1 = p.dtm1; a.dtm2 = p.dtm2; a.t1 = p.t1; a.t4 = p.t4;
+  a.sat_b0 = p.sat_b0; a.sat_uth = p.sat_uth; a.cmin = p.cmin; a.cmax = p.cmax;
+  a.nlf = p.nl;
+  a.x = x;
+  a.fstride = 1;
+  return a;
+}
+
+// Runs forward steps [n0, n1) starting from state (s1 = u_n0, s2 = u_{n0-1}).
+// Tiled: ping-pong between (s1,s2) and (o1,o2); plain: rotation through
+// three buffers.  Returns the buffers holding the final state.
+int run_forward(const Plan &p, FwdArgs a, int n0, int n1, float *bufs[6], float **fin1,
+                float **fin2, bool rec, hipStream_t s) {
+  if (p.tiled) {
+    float *i1 = bufs[0], *i2 = bufs[1], *o1 = bufs[2], *o2 = bufs[3];
+    int n = n0;
+    while (n < n1) {
+      int k = std::min(p.tun.tblock, n1 - n);
+      a.in1 = i1; a.in2 = i2; a.out1 = o1; a.out2 = o2;
+      a.n0 = n; a.nsteps = k;
+      WT_HIP(launch_fwd_tiled(p.tun.nt, p.tun.cpt, p.unit_h, p.nl, rec, a, p.LWmax, s));
+      std::swap(i1, o1);
+      std::swap(i2, o2);
+      n += k;
+    }
+    *fin1 = i1;
+    *fin2 = i2;
+  } else {
+    // u_n in r1, u_{n-1} in r2, output into r3
+    float *r1 = bufs[0], *r2 = bufs[1], *r3 = bufs[2];
+    for (int n = n0; n < n1; ++n) {
+      a.in1 = r1; a.in2 = r2; a.out1 = r3;
+      WT_HIP(launch_fwd_plain(p.unit_h, p.nl, rec, a, p.d_spmap, n, s));
+      float *t = r2; r2 = r1; r1 = r3; r3 = t;
+    }
+    *fin1 = r1;
+    *fin2 = r2;
+  }
+  return WT_OK;
+}
+
+int prepare_coef(const Plan &p, const Ws &w, const float *b, const float *c, const float *rho,
+                 float *tmp, hipStream_t s) {
+  int64_t F = (int64_t)p.F;
+  if (p.nl) {
+    WT_HIP(launch_mul_scalar(rho, p.c_nl, F, w.rhoc, s));
+  } else {
+    const float *cc = c;
+    if (p.clamp_only) {
+      WT_HIP(launch_clamp(c, p.cmin, p.cmax, F, tmp, s));
+      cc = tmp;
+    }
+    WT_HIP(launch_coef_linear(b, cc, F, p.dtm1, p.dtm2, w.A, w.Bm, w.C2, s));
+  }
+  return WT_OK;
+}
+
+}  // namespace
+
+int wt_loop_fwd(void *plan, const float *b, const float *c, const float *rho, const float *x,
+                float *y, float *fields, int32_t field_stride, int32_t with_grad, void *saved,
+                void *temp, void *stream) {
+  Plan *p = (Plan *)plan;
+  if (!p || !b || !c || !x || !temp || (with_grad && !saved) || (p->nl && !rho) ||
+      field_stride <= 0) {
+    set_error("wt_loop_fwd: invalid argument");
+    return WT_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Ws w = layout(*p, saved, temp, with_grad != 0);
+  const int64_t BF = (int64_t)p->B * p->F;
+  float *bufs[6];
+  for (int i = 0; i < 6; ++i) bufs[i] = w.st + i * BF;
+  // clamp-only: clamped c goes into the last state buffer (unused by the loop)
+  int rc = prepare_coef(*p, w, b, c, rho, bufs[5], s);
+  if (rc) return rc;
+  FwdArgs a = base_fwd(*p, w, b, c, rho, x);
+  a.y = y;
+  a.yraw = with_grad ? w.yraw : nullptr;
+  a.fields = fields;
+  a.fstride = field_stride;
+  WT_HIP(launch_fill(bufs[0], 2 * BF, 0.0f, s));  // u_0 = u_{-1} = 0 (rnn.py:39-41)
+  float *cur1 = bufs[0], *cur2 = bufs[1];
+  for (int sg = 0; sg < p->nseg; ++sg) {
+    int n0 = sg * p->K, n1 = std::min(p->T, n0 + p->K);
+    if (with_grad) {
+      float *ck = w.ck + (size_t)sg * 2 * BF;
+      WT_HIP(hipMemcpyAsync(ck, cur1, BF * sizeof(float), hipMemcpyDeviceToDevice, s));
+      WT_HIP(hipMemcpyAsync(ck + BF, cur2, BF * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    // keep the live state in bufs[0..1] so the ping-pong partners are fixed
+    float *set[6] = {cur1, cur2, nullptr, nullptr, nullptr, nullptr};
+    int nfree = 2;
+    for (int i = 0; i < 5 && nfree < 6; ++i)
+      if (bufs[i] != cur1 && bufs[i] != cur2) set[nfree++] = bufs[i];
+    float *f1, *f2;
+    rc = run_forward(*p, a, n0, n1, set, &f1, &f2, false, s);
+    if (rc) return rc;
+    cur1 = f1;
+    cur2 = f2;
+  }
+  return WT_OK;
+}
+
+int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, const float *x,
+                const float *ybar, const float *fbar, void *saved, void *temp, float *grad_c,
+                float *grad_b, float *grad_rho, float *grad_x, void *stream) {
+  Plan *p = (Plan *)plan;
+  if (!p || !b || !c || !x || !saved || !temp || (p->nl && !rho)) {
+    set_error("wt_loop_adj: invalid argument");
+    return WT_EINVAL;
+  }
+  if (p->nl) {
+    set_error("wt_loop_adj: the nonlinear cell adjoint is not available in this build");
+    return WT_EUNSUPPORTED;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Ws w = layout(*p, saved, temp, true);
+  const int64_t BF = (int64_t)p->B * p->F;
+  const int T = p->T, K = p->K;
+  bool use_tiled = p->tiled && fbar == nullptr;
+  float *st[6];
+  for (int i = 0; i < 6; ++i) st[i] = w.st + i * BF;
+  float *lam[4];
+  for (int i = 0; i < 4; ++i) lam[i] = w.lam + i * BF;
+  WT_HIP(launch_fill(w.lam, 4 * BF, 0.0f, s));  // lambda_{T+1} = lambda_{T+2} = 0
+  WT_HIP(launch_fill(w.wl, 2 * BF, 0.0f, s));
+  WT_HIP(launch_fill(w.gacc, BF, 0.0f, s));
+  WT_HIP(launch_fill(w.gsrc, (int64_t)p->B * T * std::max(p->nslot, 1), 0.0f, s));
+
+  FwdArgs fa = base_fwd(*p, w, b, c, rho, x);
+  fa.rec_lap = w.seg;
+  fa.rec_K = K;
+  AdjArgs aa{};
+  aa.B = p->B; aa.Nx = p->Nx; aa.Ny = p->Ny; aa.T = T;
+  aa.ntiles = p->ntiles; aa.tiles = p->d_tiles; aa.spx = p->d_spx; aa.taps = p->d_taps;
+  aa.ntap = p->ntap; aa.nsrcslot = std::max(p->nslot, 1);
+  aa.A = w.A; aa.Bm = w.Bm; aa.C2 = w.C2;
+  aa.s2 = p->s2; aa.dtm1 = p->dtm1; aa.dtm2 = p->dtm2; aa.t1 = p->t1; aa.t4 = p->t4;
+  aa.ybar = ybar; aa.yraw = w.yraw;
+  aa.seg_lap = w.seg; aa.seg_K = K;
+  aa.gacc = w.gacc; aa.gsrc = w.gsrc;
+
+  // adjoint state: (l1, l2) = (lambda_{m+1}, lambda_{m+2})
+  float *l1 = lam[0], *l2 = lam[1], *l3 = lam[2], *l4 = lam[3];
+  float *wl1 = w.wl, *wl2 = w.wl + BF;
+  for (int sg = p->nseg - 1; sg >= 0; --sg) {
+    int n0 = sg * K, n1 = std::min(T, n0 + K);
+    // recompute the segment from its checkpoint, recording lap(u_n)
+    float *ck = w.ck + (size_t)sg * 2 * BF;
+    WT_HIP(hipMemcpyAsync(st[0], ck, 2 * BF * sizeof(float), hipMemcpyDeviceToDevice, s));
+    fa.rec_n0 = n0;
+    float *f1, *f2;
+    int rc;
+    if (use_tiled) {
+      rc = run_forward(*p, fa, n0, n1, st, &f1, &f2, true, s);
+    } else {
+      Plan q = *p;
+      q.tiled = false;
+      rc = run_forward(q, fa, n0, n1, st, &f1, &f2, true, s);
+    }
+    if (rc) return rc;
+    aa.seg_n0 = n0;
+    // adjoint over m = n1 .. n0+1
+    int m = n1;
+    if (use_tiled) {
+      while (m > n0) {
+        int k = std::min(p->tun.tblock, m - n0);
+        aa.in1 = l1; aa.in2 = l2; aa.out1 = l3; aa.out2 = l4;
+        aa.m_hi = m; aa.nsteps = k;
+        WT_HIP(launch_adj_tiled(p->tun.nt, p->tun.cpt, p->unit_h, aa, p->LWmax, s));
+        std::swap(l1, l3);
+        std::swap(l2, l4);
+        m -= k;
+      }
+    } else {
+      for (; m > n0; --m) {
+        aa.in1 = l1; aa.in2 = l2; aa.out1 = l3;
+        WT_HIP(launch_adj_plain(p->unit_h, aa, p->d_spmap, wl1, wl2, fbar, m, s));
+        // rotate: (l1, l2, l3) <- (l3, l1, l2)
+        float *t = l2; l2 = l1; l1 = l3; l3 = t;
+        std::swap(wl1, wl2);
+      }
+      // the tiled path uses the pair (l3, l4) as outputs; keep the plain
+      // rotation consistent with it: l4 unused here
+    }
+  }
+  // for the plain path the w*lambda buffer of the tiled path is not used
+  if (!use_tiled) {
+    // nothing
+  }
+  if (grad_c) {
+    // dJ/dc = 2 c A sum_b gacc (the clamp-only case: c is time-invariant and
+    // clamped before the loop; gradient through the clamp is masked by the
+    // caller)
+    WT_HIP(launch_finalize_gc(w.gacc, p->B, (int64_t)p->F, c, w.A, grad_c, s));
+  }
+  if (grad_b) WT_HIP(launch_fill(grad_b, (int64_t)p->F, 0.0f, s));
+  if (grad_rho) WT_HIP(launch_fill(grad_rho, (int64_t)p->F, 0.0f, s));
+  if (grad_x) {
+    if (p->nslot > 0)
+      WT_HIP(launch_finalize_gx(w.gsrc, p->B, T, p->nslot, p->d_mult, grad_x, s));
+    else
+      WT_HIP(launch_fill(grad_x, (int64_t)p->B * T, 0.0f, s));
+  }
+  return WT_OK;
+}
+
+// ---------------------------------------------------------------- graphs
+
+struct Graph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+};
+
+int wt_graph_begin(void *stream) {
+  WT_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return WT_OK;
+}
+
+int wt_graph_end(void *stream, void **graph_out) {
+  if (!graph_out) {
+    set_error("wt_graph_end: null output");
+    return WT_EINVAL;
+  }
+  Graph *G = new Graph();
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &G->g);
+  if (e != hipSuccess) {
+    delete G;
+    return hip_fail(e, "hipStreamEndCapture");
+  }
+  e = hipGraphInstantiate(&G->e, G->g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(G->g);
+    delete G;
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  *graph_out = G;
+  return WT_OK;
+}
+
+int wt_graph_launch(void *graph, void *stream) {
+  if (!graph) {
+    set_error("wt_graph_launch: null graph");
+    return WT_EINVAL;
+  }
+  WT_HIP(hipGraphLaunch(((Graph *)graph)->e, (hipStream_t)stream));
+  return WT_OK;
+}
+
+int wt_graph_destroy(void *graph) {
+  Graph *G = (Graph *)graph;
+  if (!G) return WT_OK;
+  if (G->e) (void)hipGraphExecDestroy(G->e);
+  if (G->g) (void)hipGraphDestroy(G->g);
+  delete G;
+  return WT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,275 @@
+"""Material parameterisation (reference: wavetorch/geom.py).
+
+Same classes, constructor signatures, buffers and ``state_dict`` keys as the
+reference.  The PML damping field ``b`` is built once at construction
+(geom.py:63-85).  ``WaveGeometryFreeForm.c`` -- blur, tanh projection and
+the linear map to wave speed (geom.py:207-233) -- runs as gfx950 kernels
+(``wt_geom_freeform_c``) with a hand-written backward, so autograd carries
+dJ/dc from the time loop back to ``rho``.
+"""
+from __future__ import annotations
+
+from copy import deepcopy
+from typing import Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .utils import to_tensor
+
+
+class WaveGeometry(torch.nn.Module):
+    """Base geometry: domain shape, spacing h, speeds c0/c1 and the PML (geom.py:11-85)."""
+
+    def __init__(self, domain_shape: Tuple, h: float, c0: float, c1: float, abs_N: int = 20,
+                 abs_sig: float = 11, abs_p: float = 4.0):
+        super().__init__()
+        assert len(domain_shape) == 2, \
+            "len(domain_shape) must be equal to 2: only two-dimensional (2D) domains are supported"
+        self.domain_shape = tuple(domain_shape)
+        self.register_buffer("h", to_tensor(h))
+        self.register_buffer("c0", to_tensor(c0))
+        self.register_buffer("c1", to_tensor(c1))
+        self.register_buffer("abs_N", to_tensor(abs_N, dtype=torch.uint8))
+        self.register_buffer("abs_sig", to_tensor(abs_sig))
+        self.register_buffer("abs_p", to_tensor(abs_p, dtype=torch.uint8))
+        self._init_b(abs_N, abs_sig, abs_p)
+
+    def state_reconstruction_args(self):
+        return {"domain_shape": self.domain_shape,
+                "h": self.h.item(),
+                "c0": self.c0.item(),
+                "c1": self.c1.item(),
+                "abs_N": self.abs_N.item(),
+                "abs_sig": self.abs_sig.item(),
+                "abs_p": self.abs_p.item()}
+
+    def __repr__(self):
+        return "WaveGeometry shape={}, h={}".format(self.domain_shape, self.h)
+
+    def forward(self):
+        raise NotImplementedError(
+            "WaveGeometry forward() is not implemented. Although WaveGeometry is a subclass of a "
+            "torch.nn.Module, its forward() method should never be called. It only exists as a "
+            "torch.nn.Module to hook into pytorch as a component of a WaveCell.")
+
+    @property
+    def c(self):
+        raise NotImplementedError
+
+    @property
+    def b(self):
+        return self._b
+
+    @property
+    def cmax(self):
+        """Maximum wave speed for the CFL check (geom.py:55-58)."""
+        return np.max([self.c0.item(), self.c1.item()])
+
+    def constrain_to_design_region(self):
+        pass
+
+    def _init_b(self, abs_N: int, abs_sig: float, abs_p: float):
+        """Polynomial damping ramp of width abs_N on all four sides (geom.py:63-85):
+        b = sqrt(b_x^2 + b_y^2) with b_vals = abs_sig * linspace(0,1,abs_N+1)**abs_p."""
+        Nx, Ny = self.domain_shape
+        assert Nx > 2 * abs_N + 1, \
+            "The domain isn't large enough in the x-direction to fit absorbing layer. " \
+            "Nx = {} and N = {}".format(Nx, abs_N)
+        assert Ny > 2 * abs_N + 1, \
+            "The domain isn't large enough in the y-direction to fit absorbing layer. " \
+            "Ny = {} and N = {}".format(Ny, abs_N)
+        b_vals = abs_sig * torch.linspace(0.0, 1.0, abs_N + 1) ** abs_p
+        b_x = torch.zeros(Nx, Ny)
+        b_y = torch.zeros(Nx, Ny)
+        if abs_N > 0:
+            b_x[0:abs_N + 1, :] = torch.flip(b_vals, [0]).repeat(Ny, 1).transpose(0, 1)
+            b_x[(Nx - abs_N - 1):Nx, :] = b_vals.repeat(Ny, 1).transpose(0, 1)
+            b_y[:, 0:abs_N + 1] = torch.flip(b_vals, [0]).repeat(Nx, 1)
+            b_y[:, (Ny - abs_N - 1):Ny] = b_vals.repeat(Nx, 1)
+        self.register_buffer("_b", torch.sqrt(b_x ** 2 + b_y ** 2))
+
+
+class WaveGeometryHoley(WaveGeometry):
+    """Density from a sum of exponential 'holes' (geom.py:88-133)."""
+
+    def __init__(self, domain_shape: Tuple, h: float, c0: float, c1: float, abs_N: int = 20,
+                 abs_sig: float = 11, abs_p: float = 4.0, eta: float = 0.5, beta: float = 100.0,
+                 x=None, y=None, r=None):
+        super().__init__(domain_shape, h, c0, c1, abs_N, abs_sig, abs_p)
+        self.x = torch.nn.Parameter(to_tensor(x))
+        self.y = torch.nn.Parameter(to_tensor(y))
+        self.r = torch.nn.Parameter(to_tensor(r))
+        self.register_buffer("eta", to_tensor(eta))
+        self.register_buffer("beta", to_tensor(beta))
+
+    def state_reconstruction_args(self):
+        my_args = {"eta": self.eta.item(),
+                   "beta": self.beta.item(),
+                   "x": deepcopy(self.x.detach()),
+                   "y": deepcopy(self.y.detach()),
+                   "r": deepcopy(self.r.detach())}
+        return {**super().state_reconstruction_args(), **my_args}
+
+    def _rho(self):
+        eta = self.eta.item()
+        beta = self.beta.item()
+        dev = self.x.device
+        xv = torch.arange(0, self.domain_shape[0], dtype=torch.get_default_dtype(), device=dev)
+        yv = torch.arange(0, self.domain_shape[1], dtype=torch.get_default_dtype(), device=dev)
+        x, y = torch.meshgrid(xv, yv, indexing="ij")
+        rho = torch.zeros(self.domain_shape, device=dev)
+        for ri, xi, yi in zip(self.r, self.x, self.y):
+            r = torch.sqrt((x - xi).pow(2) + (y - yi).pow(2))
+            rho = rho + torch.exp(-r / ri)
+        return (np.tanh(beta * eta) + torch.tanh(beta * (rho - eta))) / (
+            np.tanh(beta * eta) + np.tanh(beta * (1 - eta)))
+
+    @property
+    def rho(self):
+        return self._rho()
+
+    @property
+    def c(self):
+        return self.c0.item() + (self.c1.item() - self.c0.item()) * self._rho()
+
+
+def _disk_pixels(radius: int) -> np.ndarray:
+    """Pixels of a disk of radius ``radius+1`` centred at (radius, radius) inside
+    the (2r+1)^2 box, by the strict rule (dr/R)^2 + (dc/R)^2 < 1 with R = radius+1
+    -- the rule of the scikit-image ``circle`` call in geom.py:149."""
+    R = radius + 1
+    m = np.zeros((2 * radius + 1, 2 * radius + 1), dtype=bool)
+    for i in range(2 * radius + 1):
+        for j in range(2 * radius + 1):
+            if ((i - radius) / R) ** 2 + ((j - radius) / R) ** 2 < 1.0:
+                m[i, j] = True
+    return m
+
+
+class _FreeFormC(torch.autograd.Function):
+    """c = c0 + (c1-c0) * P(Blur^N(rho)) on the GPU (geom.py:207-233)."""
+
+    @staticmethod
+    def forward(ctx, rho, kernel, radius, blur_N, eta, beta, c0, c1):
+        rho = rho.contiguous()
+        kernel = kernel.contiguous()
+        Nx, Ny = rho.shape
+        c = torch.empty_like(rho)
+        blur = torch.empty_like(rho)
+        work = torch.empty(2 * Nx * Ny, dtype=rho.dtype, device=rho.device)
+        s = _lib.stream_ptr(rho.device)
+        _lib.check(_lib.lib().wt_geom_freeform_c(
+            rho.data_ptr(), kernel.data_ptr(), radius, blur_N, Nx, Ny, eta, beta, c0, c1,
+            c.data_ptr(), None, blur.data_ptr(), work.data_ptr(), s), "wt_geom_freeform_c")
+        ctx.save_for_backward(blur, kernel)
+        ctx.params = (radius, blur_N, eta, beta, c0, c1)
+        return c
+
+    @staticmethod
+    def backward(ctx, grad_c):
+        blur, kernel = ctx.saved_tensors
+        radius, blur_N, eta, beta, c0, c1 = ctx.params
+        grad_c = grad_c.contiguous()
+        Nx, Ny = blur.shape
+        g = torch.empty_like(blur)
+        work = torch.empty(2 * Nx * Ny, dtype=blur.dtype, device=blur.device)
+        s = _lib.stream_ptr(blur.device)
+        _lib.check(_lib.lib().wt_geom_freeform_c_bwd(
+            grad_c.data_ptr(), blur.data_ptr(), kernel.data_ptr(), radius, blur_N, Nx, Ny, eta,
+            beta, c0, c1, g.data_ptr(), work.data_ptr(), s), "wt_geom_freeform_c_bwd")
+        return g, None, None, None, None, None, None, None
+
+
+def _require_hip(t, what):
+    if not t.is_cuda:
+        raise RuntimeError(
+            f"wavetorch_amd: {what} runs on the HIP device only; move the model with .to('cuda')")
+    if t.dtype != torch.float32:
+        raise NotImplementedError(f"wavetorch_amd: {what} computes in float32 (got {t.dtype})")
+
+
+class WaveGeometryFreeForm(WaveGeometry):
+    """Free-form density with blur and tanh projection (geom.py:136-233)."""
+
+    def __init__(self, domain_shape: Tuple, h: float, c0: float, c1: float, abs_N: int = 20,
+                 abs_sig: float = 11, abs_p: float = 4.0, eta: float = 0.5, beta: float = 100.0,
+                 design_region=None, rho='half', blur_radius: int = 1, blur_N: int = 1):
+        super().__init__(domain_shape, h, c0, c1, abs_N, abs_sig, abs_p)
+        self.register_buffer("eta", to_tensor(eta))
+        self.register_buffer("beta", to_tensor(beta))
+        self._init_design_region(design_region, domain_shape)
+        self._init_rho(rho, domain_shape)
+        m = _disk_pixels(blur_radius)
+        blur_kernel = torch.zeros((2 * blur_radius + 1, 2 * blur_radius + 1),
+                                  dtype=torch.get_default_dtype())
+        blur_kernel[torch.from_numpy(m)] = 1
+        blur_kernel = blur_kernel / blur_kernel.sum()
+        self.register_buffer("blur_kernel", blur_kernel.unsqueeze(0).unsqueeze(0))
+        self.register_buffer("blur_N", to_tensor(blur_N, dtype=torch.int))
+        # the reference stores blur_N under this name (geom.py:156); kept for
+        # state_dict compatibility
+        self.register_buffer("blur_radius", to_tensor(blur_N, dtype=torch.int))
+        self._blur_radius = int(blur_radius)
+        self.constrain_to_design_region()
+
+    def state_reconstruction_args(self):
+        my_args = {"eta": self.eta.item(),
+                   "beta": self.beta.item(),
+                   "design_region": deepcopy(self.design_region),
+                   "rho": deepcopy(self.rho.detach()),
+                   "blur_radius": self.blur_radius.item(),
+                   "blur_N": self.blur_N.item()}
+        return {**super().state_reconstruction_args(), **my_args}
+
+    def __repr__(self):
+        return super().__repr__() + ", " + str(self.design_region.sum().item()) + " DOFs"
+
+    def _init_design_region(self, design_region, domain_shape):
+        if design_region is not None:
+            assert tuple(design_region.shape) == tuple(domain_shape), \
+                "The design region shape must match domain shape; design_region.shape = {} " \
+                "domain_shape = {}".format(design_region.shape, domain_shape)
+            if type(design_region) is np.ndarray:
+                design_region = torch.from_numpy(design_region).to(torch.uint8)
+        else:
+            design_region = torch.ones(domain_shape, dtype=torch.uint8)
+        self.register_buffer("design_region", design_region)
+
+    def _init_rho(self, rho, domain_shape):
+        if isinstance(rho, torch.Tensor) | isinstance(rho, np.ndarray):
+            assert tuple(rho.shape) == tuple(domain_shape)
+            self.rho = torch.nn.Parameter(to_tensor(rho))
+        elif isinstance(rho, str):
+            if rho == 'rand':
+                self.rho = torch.nn.Parameter(torch.round(torch.rand(domain_shape)))
+            elif rho == 'half':
+                self.rho = torch.nn.Parameter(torch.ones(domain_shape) * 0.5)
+            elif rho == 'blank':
+                self.rho = torch.nn.Parameter(torch.zeros(domain_shape))
+            else:
+                raise ValueError('The domain initialization defined by `rho = %s` is invalid' % rho)
+        else:
+            raise ValueError('The domain initialization is invalid')
+
+    def constrain_to_design_region(self):
+        """rho = 0 outside the design region and wherever b > 0 (geom.py:201-205)."""
+        with torch.no_grad():
+            self.rho[self.design_region == 0] = 0.0
+            self.rho[self.b > 0] = 0.0
+
+    def _rho_model(self):
+        """Blurred and projected density P(Blur^N(rho)) (geom.py:224-229); the
+        kernels fuse it into ``c``, so it is recovered from c here."""
+        c0, c1 = self.c0.item(), self.c1.item()
+        return (self.c - c0) / (c1 - c0)
+
+    @property
+    def c(self):
+        """c0 + (c1-c0) * P(Blur^N(rho)) (geom.py:231-233), gfx950 kernels."""
+        _require_hip(self.rho, "WaveGeometryFreeForm.c")
+        k = self.blur_kernel[0, 0].to(device=self.rho.device, dtype=torch.float32)
+        return _FreeFormC.apply(self.rho, k, self._blur_radius, int(self.blur_N.item()),
+                                float(self.eta.item()), float(self.beta.item()),
+                                float(self.c0.item()), float(self.c1.item()))

@@ -1,0 +1,41 @@
+"""dtype plumbing and the loss head (reference: wavetorch/utils.py:6-36)."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+def to_tensor(x, dtype=None):
+    """utils.py:6-11: numpy arrays via from_numpy, everything else via torch.tensor,
+    in the default dtype unless given."""
+    dtype = dtype if dtype is not None else torch.get_default_dtype()
+    if type(x) is np.ndarray:
+        return torch.from_numpy(x).type(dtype=dtype)
+    else:
+        return torch.tensor(x, dtype=dtype)
+
+
+def set_dtype(dtype=None):
+    """utils.py:14-20.  The HIP kernels compute in float32; float64 is accepted
+    for module construction but the time loop raises for it."""
+    if dtype == 'float32' or dtype is None:
+        torch.set_default_dtype(torch.float32)
+    elif dtype == 'float64':
+        torch.set_default_dtype(torch.float64)
+    else:
+        raise ValueError('Unsupported data type: %s; should be either float32 or float64' % dtype)
+
+
+def window_data(X, window_length):
+    """utils.py:23-26: crop to `window_length` centred at the middle."""
+    return X[int(len(X) / 2 - window_length / 2):int(len(X) / 2 + window_length / 2)]
+
+
+def accuracy_onehot(y_pred, y_label):
+    """utils.py:29-32."""
+    return (y_pred.argmax(dim=1) == y_label).float().mean().item()
+
+
+def normalize_power(X):
+    """utils.py:35-36: probe powers normalised to sum to one over the probes."""
+    return X / torch.sum(X, dim=1, keepdim=True)
