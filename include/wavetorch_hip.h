@@ -192,6 +192,19 @@ int wt_geom_freeform_c_bwd(const float *grad_c, const float *blur_in,
                            void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Kernel timing (no counterpart in the reference, which has no profiler)    */
+/* ------------------------------------------------------------------------ */
+
+/* Enables/disables HIP-event timing of every time-loop kernel launch (events
+ * recorded on the launch stream).  Not for use under graph capture. */
+int wt_profile(int32_t enable);
+/* Synchronises the recorded events and returns, per kernel class
+ * (0 forward, 1 forward recompute with record, 2 adjoint), the summed
+ * kernel milliseconds, the number of launches and the cell-steps they
+ * processed; then clears the record.  Each array has 3 entries. */
+int wt_profile_read(double *ms, int64_t *launches, int64_t *cell_steps);
+
+/* ------------------------------------------------------------------------ */
 /* HIP graph capture of any sequence of the calls above                      */
 /* ------------------------------------------------------------------------ */
 

@@ -1,0 +1,322 @@
+"""Pins the oracle before anything is checked against it (CPU only).
+
+Which check pins which oracle function:
+
+=====================  ====================================================
+wo_laplacian_*         golden vectors from operators.py (bitwise); Dirichlet
+                       eigenvalues (closed form); quadratic field
+wo_forward_f32         golden 48-step loop run through the reference's own
+                       TimeStep.apply + probes (bitwise); torch restatement
+                       (bitwise); single step vs golden _time_step (bitwise)
+wo_forward_f64         eigenmode evolution cos(n*theta) (closed form); discrete
+                       energy conservation with b = 0 and decay with b > 0;
+                       linearity in the source amplitude
+wo_adjoint_f64         golden reference gradient (autograd through the
+                       reference's hand-written TimeStep.backward); central
+                       finite differences; dot-product identity; nonlinear
+                       chain rule vs FD and vs autograd through the
+                       restatement of cell.py:94-102
+ref_torch.TimeStepRef  golden TimeStep.backward vectors; the reference's own
+                       identity (study/utils/test_grad.py): hand backward ==
+                       autograd of _time_step
+ref_torch.pml_b etc.   closed forms of geom.py:63-85 and 149-233
+=====================  ====================================================
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.c_oracle as co
+from oracle import ref_torch as rt
+from tests.helpers import rel, small_problem
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with np.load(GOLD, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+# ------------------------------------------------------------------ Laplacian
+
+def test_laplacian_golden_bitwise(gold):
+    assert np.array_equal(co.laplacian(gold["lap_in"]), gold["lap_out"])
+    t = rt.laplacian(torch.from_numpy(gold["lap_in"]), torch.tensor(1.0)).numpy()
+    assert np.array_equal(t, gold["lap_out"])
+
+
+def test_laplacian_dirichlet_eigenvalues():
+    Nx, Ny = 11, 14
+    i = np.arange(Nx)[:, None]
+    j = np.arange(Ny)[None, :]
+    for p, q in [(1, 1), (2, 5), (7, 3)]:
+        v = np.sin(np.pi * p * (i + 1) / (Nx + 1)) * np.sin(np.pi * q * (j + 1) / (Ny + 1))
+        lam = -4.0 * (np.sin(np.pi * p / (2 * (Nx + 1))) ** 2 + np.sin(np.pi * q / (2 * (Ny + 1))) ** 2)
+        Lv = co.laplacian(v[None].astype(np.float64))[0]
+        assert np.abs(Lv - lam * v).max() < 1e-12
+
+
+def test_laplacian_quadratic_interior_and_h():
+    Nx, Ny = 9, 10
+    i, j = np.meshgrid(np.arange(Nx), np.arange(Ny), indexing="ij")
+    u = (0.5 * i ** 2 + 1.5 * j ** 2).astype(np.float64)
+    for h in (1.0, 0.5):
+        L = co.laplacian(u[None], h)[0]
+        assert np.allclose(L[1:-1, 1:-1], (1.0 + 3.0) / h ** 2, atol=1e-12)
+
+
+# ------------------------------------------------------------------ forward
+
+def test_step_golden_bitwise(gold):
+    b, c, y1, y2 = gold["step_b"], gold["step_c"], gold["step_y1"], gold["step_y2"]
+    y = rt.time_step(torch.from_numpy(b), torch.from_numpy(c), torch.from_numpy(y1),
+                     torch.from_numpy(y2), torch.tensor(0.707), torch.tensor(1.0)).numpy()
+    assert np.array_equal(y, gold["step_y"])
+
+
+def test_loop_golden_bitwise(gold):
+    y, _ = co.forward(gold["loop_b"], gold["loop_c"], gold["loop_x"], float(gold["loop_dt"]), 1.0,
+                      gold["loop_src"], gold["loop_taps"])
+    assert np.array_equal(y, gold["loop_y"])
+
+
+def test_loop_torch_restatement_bitwise():
+    p = small_problem(T=80)
+    src = [(torch.tensor(s[0]), torch.tensor(s[1])) for s in p["src"]]
+    probes = [(torch.tensor(t[0]), torch.tensor(t[1]), bool(t[2])) for t in p["taps"]]
+    yt = rt.rnn_forward(torch.from_numpy(p["x"]), c=torch.from_numpy(p["c"]), rho=None,
+                        b_geom=torch.from_numpy(p["b"]), dt=torch.tensor(0.707),
+                        h=torch.tensor(1.0), sources=src, probes=probes).numpy()
+    yc, _ = co.forward(p["b"], p["c"], p["x"], 0.707, 1.0, p["src"], p["taps"])
+    assert np.array_equal(yc, yt)
+
+
+def test_eigenmode_evolution_closed_form():
+    """b = 0, uniform c: u_n = cos(n theta) v with cos(theta) = 1 + (c dt)^2 lam / 2."""
+    Nx, Ny, cval, dt = 12, 15, 0.8, 0.7
+    i = np.arange(Nx)[:, None]
+    j = np.arange(Ny)[None, :]
+    p, q = 2, 3
+    v = np.sin(np.pi * p * (i + 1) / (Nx + 1)) * np.sin(np.pi * q * (j + 1) / (Ny + 1))
+    lam = -4.0 * (np.sin(np.pi * p / (2 * (Nx + 1))) ** 2 + np.sin(np.pi * q / (2 * (Ny + 1))) ** 2)
+    th = np.arccos(1 + (cval * dt) ** 2 * lam / 2)
+    b = torch.zeros(Nx, Ny, dtype=torch.float64)
+    c = torch.full((Nx, Ny), cval, dtype=torch.float64)
+    y2 = torch.from_numpy(v)[None]                 # u_0
+    y1 = torch.from_numpy(np.cos(th) * v)[None]    # u_1
+    for n in range(2, 60):
+        y = rt.time_step(b, c, y1, y2, torch.tensor(dt, dtype=torch.float64),
+                         torch.tensor(1.0, dtype=torch.float64))
+        y2, y1 = y1, y
+        assert np.abs(y[0].numpy() - np.cos(n * th) * v).max() < 1e-11
+
+
+def _energy(f, c, dt):
+    """E_n = sum |u_{n+1}-u_n|^2 / c^2 - dt^2 <u_{n+1}, L u_n>  (conserved for b = 0)."""
+    E = []
+    for n in range(f.shape[1] - 1):
+        un, un1 = f[:, n], f[:, n + 1]
+        L = co.laplacian(un)
+        E.append(np.sum((un1 - un) ** 2 / c ** 2) - dt ** 2 * np.sum(un1 * L))
+    return np.array(E)
+
+
+def test_energy_invariant_and_pml_decay():
+    p = small_problem(T=200, abs_N=5)
+    x = p["x"].astype(np.float64)
+    x[:, 100:] = 0.0  # source off after step 100
+    b0 = np.zeros_like(p["b"], dtype=np.float64)
+    _, f = co.forward(b0, p["c"], x, 0.6, 1.0, p["src"], p["taps"], fields=True, dtype=np.float64)
+    E = _energy(f, p["c"].astype(np.float64), 0.6)[101:]
+    assert np.abs(E - E[0]).max() < 1e-10 * np.abs(E[0])
+    _, f = co.forward(p["b"], p["c"], x, 0.6, 1.0, p["src"], p["taps"], fields=True,
+                      dtype=np.float64)
+    E = _energy(f, p["c"].astype(np.float64), 0.6)[101:]
+    assert E[-1] < 0.9 * E[0] and np.all(np.diff(E) <= 1e-12 * E[0])
+
+
+def test_linearity_in_source_amplitude():
+    p = small_problem(T=60)
+    x = p["x"].astype(np.float64)
+    y1, f1 = co.forward(p["b"], p["c"], x, 0.707, 1.0, p["src"], p["taps"], fields=True,
+                        dtype=np.float64)
+    y3, f3 = co.forward(p["b"], p["c"], 3 * x, 0.707, 1.0, p["src"], p["taps"], fields=True,
+                        dtype=np.float64)
+    assert np.abs(f3 - 3 * f1).max() < 1e-12 * np.abs(f1).max()
+
+
+# ------------------------------------------------------------------ adjoint
+
+def test_timestep_backward_golden(gold):
+    args = [torch.from_numpy(gold[k]).clone().requires_grad_(True)
+            for k in ("step_b", "step_c", "step_y1", "step_y2")]
+    y = rt.TimeStepRef.apply(*args, torch.tensor(0.707), torch.tensor(1.0))
+    y.backward(torch.from_numpy(gold["step_g"]))
+    for a, k in zip(args, ("step_gb", "step_gc", "step_gy1", "step_gy2")):
+        assert np.array_equal(a.grad.numpy(), gold[k]), k
+
+
+def test_reference_identity_hand_backward_equals_autograd():
+    """study/utils/test_grad.py:61-84 as an assertion, in fp64."""
+    g = torch.Generator().manual_seed(3)
+    params = [torch.rand(5, 5, generator=g, dtype=torch.float64, requires_grad=True),
+              torch.rand(5, 5, generator=g, dtype=torch.float64, requires_grad=True),
+              torch.rand(1, 5, 5, generator=g, dtype=torch.float64, requires_grad=True),
+              torch.rand(1, 5, 5, generator=g, dtype=torch.float64, requires_grad=True)]
+    dt = torch.tensor(0.1, dtype=torch.float64)
+    h = torch.tensor(0.25, dtype=torch.float64)
+    rt.time_step(*params, dt, h).sum().backward()
+    ad = [q.grad.clone() for q in params]
+    for q in params:
+        q.grad = None
+    rt.TimeStepRef.apply(*params, dt, h).sum().backward()
+    for q, a in zip(params, ad):
+        assert rel(q.grad.numpy(), a.numpy()) < 1e-13
+
+
+def test_adjoint_vs_golden_reference_gradient(gold):
+    g = co.adjoint(gold["loop_b"], gold["loop_c"], gold["loop_x"], float(gold["loop_dt"]), 1.0,
+                   gold["loop_src"], gold["loop_taps"], ybar=gold["loop_w"])
+    assert rel(g["gc"], gold["loop_gc"]) < 1e-5
+    assert rel(g["gx"], gold["loop_gx"]) < 1e-5
+
+
+def _J(p, c=None, x=None, b=None, w=None, nl=None):
+    y, _ = co.forward(p["b"] if b is None else b, p["c"] if c is None else c,
+                      p["x"] if x is None else x, p["dt"], 1.0, p["src"], p["taps"], nl=nl,
+                      dtype=np.float64)
+    return float(np.sum(y * w))
+
+
+def test_adjoint_central_fd():
+    p = small_problem(Nx=16, Ny=14, B=2, T=50, abs_N=3)
+    p["dt"] = 0.65
+    rng = np.random.default_rng(7)
+    w = rng.standard_normal((2, 50, 3))
+    g = co.adjoint(p["b"], p["c"], p["x"], p["dt"], 1.0, p["src"], p["taps"], ybar=w)
+    c0 = p["c"].astype(np.float64)
+    for (i, j) in [(5, 6), (8, 4), (10, 10), (3, 3)]:
+        e = 1e-6
+        cp, cm = c0.copy(), c0.copy()
+        cp[i, j] += e
+        cm[i, j] -= e
+        fd = (_J(p, c=cp, w=w) - _J(p, c=cm, w=w)) / (2 * e)
+        assert abs(fd - g["gc"][i, j]) < 1e-6 * max(1.0, abs(fd)), (i, j, fd, g["gc"][i, j])
+    x0 = p["x"].astype(np.float64)
+    for (bb, n) in [(0, 3), (1, 20), (0, 45)]:
+        e = 1e-6
+        xp, xm = x0.copy(), x0.copy()
+        xp[bb, n] += e
+        xm[bb, n] -= e
+        fd = (_J(p, x=xp, w=w) - _J(p, x=xm, w=w)) / (2 * e)
+        assert abs(fd - g["gx"][bb, n]) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_adjoint_dot_product():
+    """<J dc, w> == <dc, J^T w>: forward is linear in x, so for x the
+    identity is exact: sum(y(x) * w) = <x, gx>."""
+    p = small_problem(Nx=16, Ny=14, B=2, T=50, abs_N=3)
+    rng = np.random.default_rng(9)
+    taps = [[t[0], t[1], 0] for t in p["taps"]]  # amplitude probes: y linear in x
+    w = rng.standard_normal((2, 50, 3))
+    y, _ = co.forward(p["b"], p["c"], p["x"], 0.707, 1.0, p["src"], taps, dtype=np.float64)
+    g = co.adjoint(p["b"], p["c"], p["x"], 0.707, 1.0, p["src"], taps, ybar=w)
+    lhs = np.sum(y * w)
+    rhs = np.sum(p["x"].astype(np.float64) * g["gx"])
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+NL = dict(sat_b0=0.5, sat_uth=0.1 ** 0.5, c_nl=0.05)
+
+
+@pytest.mark.parametrize("which", ["sat", "kerr", "both", "both+clamp"])
+def test_nonlinear_adjoint_fd(which):
+    p = small_problem(Nx=16, Ny=14, B=2, T=50, abs_N=3, amp=0.3)
+    p["dt"] = 0.6
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0, 1, (16, 14))
+    nl = dict(rho=rho)
+    if which in ("sat", "both", "both+clamp"):
+        nl.update(sat_b0=0.5, sat_uth=0.1 ** 0.5)
+    if which in ("kerr", "both", "both+clamp"):
+        nl.update(c_nl=0.05)
+    if which == "both+clamp":
+        nl.update(clamp=(0.55, 0.9))
+    w = rng.standard_normal((2, 50, 3))
+    g = co.adjoint(p["b"], p["c"], p["x"], p["dt"], 1.0, p["src"], p["taps"], ybar=w, nl=nl)
+    for (i, j) in [(5, 6), (8, 4), (10, 9)]:
+        e = 1e-6
+        for key, field in (("gc", "c"), ("gb", "b"), ("grho", "rho")):
+            base = (rho if field == "rho" else p[field]).astype(np.float64)
+            fp, fm = base.copy(), base.copy()
+            fp[i, j] += e
+            fm[i, j] -= e
+            if field == "rho":
+                Jp = _J(p, w=w, nl={**nl, "rho": fp})
+                Jm = _J(p, w=w, nl={**nl, "rho": fm})
+            else:
+                Jp = _J(p, w=w, nl=nl, **{field: fp})
+                Jm = _J(p, w=w, nl=nl, **{field: fm})
+            fd = (Jp - Jm) / (2 * e)
+            assert abs(fd - g[key][i, j]) < 2e-6 * max(1.0, abs(fd)), (which, key, i, j)
+
+
+def test_nonlinear_adjoint_vs_autograd_restatement():
+    p = small_problem(Nx=16, Ny=14, B=2, T=40, abs_N=3, amp=0.3)
+    rng = np.random.default_rng(13)
+    rho = rng.uniform(0, 1, (16, 14))
+    D = torch.float64
+    c = torch.from_numpy(p["c"]).to(D).requires_grad_(True)
+    b = torch.from_numpy(p["b"]).to(D).requires_grad_(True)
+    r = torch.from_numpy(rho).requires_grad_(True)
+    x = torch.from_numpy(p["x"]).to(D).requires_grad_(True)
+    src = [(torch.tensor(s[0]), torch.tensor(s[1])) for s in p["src"]]
+    probes = [(torch.tensor(t[0]), torch.tensor(t[1]), bool(t[2])) for t in p["taps"]]
+    y = rt.rnn_forward(x, c=c, rho=r, b_geom=b, dt=torch.tensor(0.6, dtype=D),
+                       h=torch.tensor(1.0, dtype=D), sources=src, probes=probes,
+                       satdamp_b0=torch.tensor(0.5, dtype=D),
+                       satdamp_uth=torch.tensor(0.1 ** 0.5, dtype=D),
+                       c_nl=torch.tensor(0.05, dtype=D))
+    w = rng.standard_normal(tuple(y.shape))
+    (y * torch.from_numpy(w)).sum().backward()
+    g = co.adjoint(p["b"], p["c"], p["x"], 0.6, 1.0, p["src"], p["taps"], ybar=w,
+                   nl=dict(rho=rho, **NL))
+    assert rel(g["gc"], c.grad.numpy()) < 1e-12
+    assert rel(g["gb"], b.grad.numpy()) < 1e-12
+    assert rel(g["grho"], r.grad.numpy()) < 1e-12
+    assert rel(g["gx"], x.grad.numpy()) < 1e-12
+
+
+def test_saturable_damping_golden(gold):
+    s = rt.saturable_damping(torch.from_numpy(gold["sat_u"]), torch.tensor(0.1 ** 0.5),
+                             torch.tensor(0.5)).numpy()
+    assert np.array_equal(s, gold["sat_out"])
+
+
+# ------------------------------------------------------------------ geometry
+
+def test_pml_closed_form():
+    N, sig, pw, Nx, Ny = 20, 1.0, 2, 100, 100
+    b = rt.pml_b((Nx, Ny), N, sig, pw).numpy().astype(np.float64)
+    k = np.arange(N + 1)
+    ramp = sig * (k / N) ** pw
+    # left edge column in the middle row: b = ramp reversed, 0 at index N
+    assert np.allclose(b[N - k, Ny // 2], ramp, rtol=1e-6, atol=1e-7)
+    assert np.allclose(b[Nx - N - 1 + k, Ny // 2], ramp, rtol=1e-6, atol=1e-7)
+    assert np.all(b[N + 1:Nx - N - 1, N + 1:Ny - N - 1] == 0)
+    assert abs(b[0, 0] - np.sqrt(2) * sig) < 1e-6
+
+
+def test_blur_kernel_and_projection_closed_forms():
+    k = rt.blur_kernel(1).numpy()[0, 0]
+    assert np.allclose(k, np.full((3, 3), 1 / 9), atol=1e-7)
+    # r = 3: (a^2 + b^2) < 16 over [-3,3]^2 drops only the four corners
+    k2 = rt.disk_mask(3)
+    assert k2.sum() == 45 and not k2[0, 0] and k2[0, 1] and k2[3, 3]
+    x = torch.tensor([0.0, 0.5, 1.0], dtype=torch.float64)
+    P = rt.apply_projection(x, 0.5, 100.0).numpy()
+    assert np.allclose(P, [0.0, 0.5, 1.0], atol=1e-12)

@@ -73,6 +73,9 @@ _SIGS = {
                                           c_f32, c_f32, c_p, c_p, c_p, c_p, c_p]),
     "wt_geom_freeform_c_bwd": (ctypes.c_int, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_i32, c_f32,
                                               c_f32, c_f32, c_f32, c_p, c_p, c_p]),
+    "wt_profile": (ctypes.c_int, [c_i32]),
+    "wt_profile_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64),
+                                       ctypes.POINTER(c_i64)]),
     "wt_graph_begin": (ctypes.c_int, [c_p]),
     "wt_graph_end": (ctypes.c_int, [c_p, ctypes.POINTER(c_p)]),
     "wt_graph_launch": (ctypes.c_int, [c_p, c_p]),

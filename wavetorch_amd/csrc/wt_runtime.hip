@@ -66,6 +66,52 @@ hipError_t launch_mul_scalar(const float *in, float v, int64_t n, float *out, hi
 hipError_t launch_clamp(const float *in, float lo, float hi, int64_t n, float *out,
                         hipStream_t s);
 
+// ---------------------------------------------------------------- profiling
+//
+// Optional per-kernel-class timing with HIP events recorded on the launch
+// stream around every launch of the loop kernels (enabled by wt_profile).
+// Classes: 0 forward, 1 forward-with-record (segment recompute), 2 adjoint.
+struct Prof {
+  std::mutex mu;
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+  std::vector<int64_t> work[3];  // cell-steps of each launch
+  std::vector<hipEvent_t> pool;
+};
+static Prof g_prof;
+
+static hipEvent_t prof_event() {
+  hipEvent_t e = nullptr;
+  if (!g_prof.pool.empty()) {
+    e = g_prof.pool.back();
+    g_prof.pool.pop_back();
+  } else {
+    (void)hipEventCreate(&e);
+  }
+  return e;
+}
+
+struct ProfScope {
+  int cls;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  int64_t work;
+  ProfScope(int c, hipStream_t st, int64_t w) : cls(c), s(st), work(w) {
+    if (!g_prof.on) return;
+    std::lock_guard<std::mutex> g(g_prof.mu);
+    a = prof_event();
+    (void)hipEventRecord(a, s);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    std::lock_guard<std::mutex> g(g_prof.mu);
+    hipEvent_t b = prof_event();
+    (void)hipEventRecord(b, s);
+    g_prof.ev[cls].push_back({a, b});
+    g_prof.work[cls].push_back(work);
+  }
+};
+
 constexpr int SRC_SLOTS_HOST = 4;  // must match wt_tiled.hip
 constexpr int SPC_SLOTS_HOST = 2;
 constexpr size_t LDS_LIMIT = 160 * 1024;
@@ -597,6 +643,7 @@ int run_forward(const Plan &p, FwdArgs a, int n0, int n1, float *bufs[6], float 
       int k = std::min(p.tun.tblock, n1 - n);
       a.in1 = i1; a.in2 = i2; a.out1 = o1; a.out2 = o2;
       a.n0 = n; a.nsteps = k;
+      ProfScope ps(rec ? 1 : 0, s, (int64_t)p.B * p.Nx * p.Ny * k);
       WT_HIP(launch_fwd_tiled(p.tun.nt, p.tun.cpt, p.unit_h, p.nl, rec, a, p.LWmax, s));
       std::swap(i1, o1);
       std::swap(i2, o2);
@@ -609,6 +656,7 @@ int run_forward(const Plan &p, FwdArgs a, int n0, int n1, float *bufs[6], float 
     float *r1 = bufs[0], *r2 = bufs[1], *r3 = bufs[2];
     for (int n = n0; n < n1; ++n) {
       a.in1 = r1; a.in2 = r2; a.out1 = r3;
+      ProfScope ps(rec ? 1 : 0, s, (int64_t)p.B * p.Nx * p.Ny);
       WT_HIP(launch_fwd_plain(p.unit_h, p.nl, rec, a, p.d_spmap, n, s));
       float *t = r2; r2 = r1; r1 = r3; r3 = t;
     }
@@ -747,6 +795,7 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
         int k = std::min(p->tun.tblock, m - n0);
         aa.in1 = l1; aa.in2 = l2; aa.out1 = l3; aa.out2 = l4;
         aa.m_hi = m; aa.nsteps = k;
+        ProfScope ps(2, s, (int64_t)p->B * p->Nx * p->Ny * k);
         WT_HIP(launch_adj_tiled(p->tun.nt, p->tun.cpt, p->unit_h, aa, p->LWmax, s));
         std::swap(l1, l3);
         std::swap(l2, l4);
@@ -755,6 +804,7 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
     } else {
       for (; m > n0; --m) {
         aa.in1 = l1; aa.in2 = l2; aa.out1 = l3;
+        ProfScope ps(2, s, (int64_t)p->B * p->Nx * p->Ny);
         WT_HIP(launch_adj_plain(p->unit_h, aa, p->d_spmap, wl1, wl2, fbar, m, s));
         // rotate: (l1, l2, l3) <- (l3, l1, l2)
         float *t = l2; l2 = l1; l1 = l3; l3 = t;
@@ -781,6 +831,40 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
       WT_HIP(launch_finalize_gx(w.gsrc, p->B, T, p->nslot, p->d_mult, grad_x, s));
     else
       WT_HIP(launch_fill(grad_x, (int64_t)p->B * T, 0.0f, s));
+  }
+  return WT_OK;
+}
+
+// ---------------------------------------------------------------- profiling
+
+int wt_profile(int32_t enable) {
+  std::lock_guard<std::mutex> g(g_prof.mu);
+  g_prof.on = enable != 0;
+  return WT_OK;
+}
+
+int wt_profile_read(double *ms, int64_t *launches, int64_t *cell_steps) {
+  std::lock_guard<std::mutex> g(g_prof.mu);
+  for (int c = 0; c < 3; ++c) {
+    double tot = 0.0;
+    int64_t w = 0;
+    for (size_t i = 0; i < g_prof.ev[c].size(); ++i) {
+      auto &pr = g_prof.ev[c][i];
+      hipError_t e = hipEventSynchronize(pr.second);
+      if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize(profile)");
+      float t = 0.f;
+      e = hipEventElapsedTime(&t, pr.first, pr.second);
+      if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime(profile)");
+      tot += t;
+      w += g_prof.work[c][i];
+      g_prof.pool.push_back(pr.first);
+      g_prof.pool.push_back(pr.second);
+    }
+    if (ms) ms[c] = tot;
+    if (launches) launches[c] = (int64_t)g_prof.ev[c].size();
+    if (cell_steps) cell_steps[c] = w;
+    g_prof.ev[c].clear();
+    g_prof.work[c].clear();
   }
   return WT_OK;
 }
