@@ -119,6 +119,9 @@ typedef struct wt_tuning {
   int32_t nt;        /* threads per workgroup: 256, 512 or 1024 */
   int32_t cpt;       /* cells per thread (compile-time variants) */
   int32_t ck_every;  /* checkpoint interval K in steps (0 = 64) */
+  int32_t kind;      /* tiled kernel family: 0 = column segments (cpt = rows
+                        per thread), 1 = flat cell list */
+  int32_t tcols;     /* column tiles per sample (column-segment family; 0 = 1) */
 } wt_tuning;
 
 /* Creates a plan for `cfg` at batch B and length T: validates the

@@ -20,10 +20,19 @@ pytestmark = pytest.mark.gpu
 TUNINGS = [
     {"plain": 1},
     {},                                              # library heuristic
-    {"strips": 1, "tblock": 64, "nt": 1024, "cpt": 8},
-    {"strips": 3, "tblock": 4, "nt": 1024, "cpt": 8},
-    {"strips": 2, "tblock": 16, "nt": 512, "cpt": 16},
-    {"strips": 5, "tblock": 7, "nt": 256, "cpt": 32},
+    # flat cell-list family
+    {"kind": 1, "strips": 1, "tblock": 64, "nt": 1024, "cpt": 8},
+    {"kind": 1, "strips": 3, "tblock": 4, "nt": 1024, "cpt": 8},
+    {"kind": 1, "strips": 2, "tblock": 16, "nt": 512, "cpt": 16},
+    {"kind": 1, "strips": 5, "tblock": 7, "nt": 256, "cpt": 32},
+    # column-segment family (cpt = rows per thread)
+    {"kind": 0, "strips": 1, "tblock": 64, "nt": 1024, "cpt": 4},
+    {"kind": 0, "strips": 2, "tblock": 9, "nt": 1024, "cpt": 6},
+    {"kind": 0, "strips": 3, "tblock": 5, "nt": 1024, "cpt": 12},
+    {"kind": 0, "strips": 4, "tblock": 16, "nt": 1024, "cpt": 10},
+    {"kind": 0, "strips": 6, "tblock": 3, "nt": 512, "cpt": 16},
+    {"kind": 0, "strips": 2, "tcols": 2, "tblock": 8, "nt": 1024, "cpt": 8},
+    {"kind": 0, "strips": 3, "tcols": 3, "tblock": 6, "nt": 1024, "cpt": 4},
 ]
 
 
@@ -67,3 +76,29 @@ def test_adjoint_vs_oracle(tun, cuda):
     g = co.adjoint(p["b"], p["c"], p["x"], p["dt"], p["h"], p["src"], p["taps"], ybar=w)
     assert rel(c.grad.cpu().numpy(), g["gc"]) < 1e-4
     assert rel(x.grad.cpu().numpy(), g["gx"]) < 1e-4
+
+
+def test_train_shape_bitwise_and_grad(cuda):
+    """The 150x150 training geometry (20-cell PML, source (40,75), probes at
+    row 110) at batch 4 and 200 steps, library-chosen tiling."""
+    from oracle import ref_torch
+    rng = np.random.default_rng(21)
+    N, B, T = 150, 4, 200
+    b = ref_torch.pml_b((N, N), 20, 1.0, 2).numpy()
+    c = np.ones((N, N), np.float32)
+    c[20:130, 20:130] = rng.uniform(0.5, 1.0, (110, 110)).astype(np.float32)
+    n = np.arange(T)
+    x = np.stack([np.sin(2 * np.pi * (0.03 + 0.01 * k) * n) for k in range(B)]).astype(np.float32)
+    p = dict(b=b, c=c, x=x, src=[[40, 75, 0]], taps=[[110, 55, 1], [110, 75, 1], [110, 95, 1]],
+             dt=0.707, h=1.0, Nx=N, Ny=N, B=B, T=T)
+    for tun in ({}, {"kind": 1}, {"plain": 1},
+                {"strips": 2, "tcols": 2, "tblock": 12, "nt": 1024, "cpt": 8}):
+        print("case", tun, flush=True)
+        plan, xt, ct, y = _run(p, tun, cuda, grad=True, K=64)
+        yo, _ = co.forward(b, c, x, 0.707, 1.0, p["src"], p["taps"])
+        assert np.array_equal(y.detach().cpu().numpy(), yo), plan.tuning
+        w = rng.standard_normal(y.shape).astype(np.float32)
+        (y * torch.from_numpy(w).to(cuda)).sum().backward()
+        g = co.adjoint(b, c, x, 0.707, 1.0, p["src"], p["taps"], ybar=w)
+        assert rel(ct.grad.cpu().numpy(), g["gc"]) < 1e-4, plan.tuning
+        assert rel(xt.grad.cpu().numpy(), g["gx"]) < 1e-4, plan.tuning

@@ -41,7 +41,8 @@ class WtConfig(ctypes.Structure):
 class WtTuning(ctypes.Structure):
     """Mirror of ``wt_tuning``."""
     _fields_ = [("plain", c_i32), ("strips", c_i32), ("tblock", c_i32), ("nt", c_i32),
-                ("cpt", c_i32), ("ck_every", c_i32)]
+                ("cpt", c_i32), ("ck_every", c_i32), ("kind", c_i32),
+                ("tcols", c_i32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

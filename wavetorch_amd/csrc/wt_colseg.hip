@@ -1,0 +1,603 @@
+// wt_colseg.hip -- column-segment, temporally blocked WaveRNN kernels (gfx950).
+//
+// Same contract as the flat kernels of wt_tiled.hip (one workgroup advances
+// one region = tile + halo of one sample by `nsteps` steps, no inter-workgroup
+// synchronisation), with a thread mapping that keeps vertical neighbours in
+// registers:
+//
+//   thread t -> column c = t % RW, segment k = t / RW; it owns the R cells
+//   (rows k*R .. k*R+R-1, column c) of the region.  u_n and u_{n-1} of those
+//   cells, and their coefficients, live in VGPRs; up/down neighbours are the
+//   thread's own registers except at the segment ends.
+//
+//   LDS holds the field column-major: cell (row, col) at (col+1)*CS + row+1,
+//   CS odd (consecutive lanes = consecutive columns hit distinct banks), with
+//   a zero column on each side and a zero row above/below.  Per cell and step
+//   one LDS write (the new value) and two reads (left, right neighbours);
+//   every LDS address is a per-thread base plus an immediate offset.
+//
+//   Halo cone: rows whose value can no longer reach the output rows before
+//   the end of the launch are not computed (whole segments are skipped).
+//
+// Forward per cell, in the reference's operation order (cell.py:12-17):
+//     s = up + lf;  s = fma(-4, ctr, s)  (== s + (-4*ctr): the product is
+//     exact);  s = s + rt;  lap = s + dn
+//     v = A*(((s2*ctr) - (Bm*u_prev)) + (C2*lap))   then + x_n at sources.
+// Adjoint per cell (linear cell, row strips):
+//     lam_m = a1*lam_{m+1} + L(w*lam_{m+1}) - a2*lam_{m+2} (+ probe seed)
+//     gacc += lap(u_{m-1}) * lam_m
+// Cells with a source pixel or a probe tap take a slower code path that only
+// the waves containing them execute (wave-uniform branch).
+#include "wt_internal.h"
+
+namespace wt {
+
+// Largest number of steps one launch of these kernels may take (the input
+// samples of a launch are staged in LDS).
+constexpr int CS_MAX_TBLOCK = 256;
+
+struct CsGeom {
+  int R0, RH, C0, RW;     // region
+  int or0, or1, oc0, oc1; // output rows / cols (domain)
+  int k, c;               // segment, column of this thread
+  bool active;            // thread owns cells
+  int CS;                 // LDS column stride (odd)
+  int base;               // LDS index of this thread's first cell
+};
+
+template <int R>
+__device__ __forceinline__ CsGeom cs_geom(const int32_t *td, int tid) {
+  CsGeom g;
+  g.R0 = td[0]; g.RH = td[1]; g.C0 = td[2]; g.RW = td[3];
+  g.or0 = td[4]; g.or1 = td[5]; g.oc0 = td[6]; g.oc1 = td[7];
+  const int nseg = (g.RH + R - 1) / R;
+  g.k = tid / g.RW;
+  g.c = tid - g.k * g.RW;
+  g.active = g.k < nseg;
+  g.CS = (nseg * R + 2) | 1;
+  g.base = (g.c + 1) * g.CS + g.k * R + 1;
+  return g;
+}
+
+// Rows [lo, hi) of the region that still influence the output after step s
+// of a launch of n steps.
+__device__ __forceinline__ void cone(const CsGeom &g, int s, int n, int &lo, int &hi) {
+  const int rem = n - 1 - s;
+  const int olo = g.or0 - g.R0, ohi = g.or1 - g.R0;
+  lo = olo > 0 ? max(0, olo - rem) : 0;
+  hi = ohi < g.RH ? min(g.RH, ohi + rem) : g.RH;
+}
+
+// Raw buffer access with hardware range checking (gfx9 descriptor word 3 =
+// 0x00020000): a load outside [0, num_records) bytes returns 0 and a store
+// there is dropped.  The slices below put the buffer base at the first
+// output row of the tile, so rows above the output rows give negative byte
+// offsets (huge as unsigned) and rows below give offsets >= num_records:
+// both fall outside the range without any per-row predicate.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t cs_rsrc(const float *p, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float cs_bload(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0));
+}
+__device__ __forceinline__ void cs_bstore(__amdgpu_buffer_rsrc_t rs, uint32_t off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, (int)off, 0, 0);
+}
+
+// Byte offset of this thread's first cell relative to the first output row
+// of its tile (column C0 + c); an out-of-range marker if the thread's column
+// is outside the tile's output columns.
+__device__ __forceinline__ uint32_t cs_out_voff(const CsGeom &g, int row0, int Ny) {
+  const int gc = g.C0 + g.c;
+  if (!g.active || gc < g.oc0 || gc >= g.oc1) return 0x80000000u;
+  return (uint32_t)(((g.R0 + row0 - g.or0) * Ny + gc) * 4);
+}
+
+// ------------------------------------------------------------------ forward
+
+// Writes the output tile of the field in LDS buffer Q to fields[(b,idx)]:
+// a flat loop over the tile (used only at snapshot steps).
+__device__ __forceinline__ void cs_snapshot(const CsGeom &g, const float *Q, float *fo, int Ny) {
+  const int th = g.or1 - g.or0, tw = g.oc1 - g.oc0;
+  for (int i = threadIdx.x; i < th * tw; i += blockDim.x) {
+    const int rr = i / tw, cc = i - (i / tw) * tw;
+    const int row = g.or0 - g.R0 + rr, col = g.oc0 - g.C0 + cc;
+    fo[(int64_t)(g.or0 + rr) * Ny + (g.oc0 + cc)] = Q[(col + 1) * g.CS + row + 1];
+  }
+}
+
+// One forward step of this thread's segment; REC records lap(u_n) (and u_n
+// for the nonlinear cell) of the output rows through range-checked buffers.
+template <int R, int NL, bool REC>
+__device__ __forceinline__ void cs_fwd_rows(const FwdArgs &a, const float *__restrict__ P,
+                                            float *__restrict__ Q,
+                                            float (&Uc)[R], float (&Up)[R], const float (&KA)[R],
+                                            const float (&KB)[R], const float (&KC)[R],
+                                            const float (&KD)[R], const float (&KS)[R],
+                                            int base, int CS, uint32_t vmask, float xv,
+                                            __amdgpu_buffer_rsrc_t rsl, __amdgpu_buffer_rsrc_t rsu,
+                                            uint32_t voff, int Ny) {
+  const int lfb = base - CS, rtb = base + CS;
+  float upv = P[base - 1];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const float ct = Uc[r];
+    const float dn = (r < R - 1) ? Uc[r + 1] : P[base + R];
+    const float lf = P[lfb + r], rt = P[rtb + r];
+    float sacc = upv + lf;
+    sacc = __builtin_fmaf(-4.0f, ct, sacc);
+    sacc = sacc + rt;
+    const float lap = sacc + dn;
+    float A, Bm, C2;
+    if (NL == 0) {
+      A = KA[r]; Bm = KB[r]; C2 = KC[r];
+    } else {
+      float bb = KA[r], cc = KB[r];
+      if (a.nlf & 1) {
+        float q = ct / a.sat_uth;
+        q = fabsf(q);
+        q = q * q;
+        q = q + 1.0f;
+        q = a.sat_b0 / q;
+        bb = bb + KC[r] * q;
+      }
+      if (a.nlf & 2) cc = cc + KD[r] * (ct * ct);
+      if (a.nlf & 4) cc = fminf(fmaxf(cc, a.cmin), a.cmax);
+      const float bd = bb * a.dtm1;
+      A = 1.0f / (a.dtm2 + bd);
+      Bm = a.dtm2 - bd;
+      C2 = cc * cc;
+      if (!((vmask >> r) & 1u)) A = 0.0f;
+    }
+    const float p1 = a.s2 * ct;
+    const float p2 = Bm * Up[r];
+    const float p3 = C2 * lap;
+    float v = A * ((p1 - p2) + p3);
+    // source: weight 1 at source pixels, 0 elsewhere; fma(1, x, v) rounds
+    // once, exactly like the reference's v + x (source.py:22)
+    v = __builtin_fmaf(KS[r], xv, v);
+    upv = ct;
+    Up[r] = v;
+    Q[base + r] = v;
+    if (REC) {
+      const uint32_t o = voff + (uint32_t)(r * Ny * 4);
+      cs_bstore(rsl, o, lap);
+      if (NL) cs_bstore(rsu, o, ct);
+    }
+  }
+}
+
+template <int NT, int R, int NL, bool REC>
+__global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
+  extern __shared__ float smem[];
+  const int tid = threadIdx.x;
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = w / a.ntiles, ti = w - (w / a.ntiles) * a.ntiles;
+  const int32_t *td = a.tiles + ti * TILE_W;
+  const CsGeom g = cs_geom<R>(td, tid);
+  const int NBUF = (g.RW + 2) * g.CS;
+  float *buf0 = smem, *buf1 = smem + NBUF;
+  float *xs = smem + 2 * NBUF;  // the input samples of this launch
+  const int Ny = a.Ny;
+  const int64_t F = (int64_t)a.Nx * Ny;
+  const int64_t boff = (int64_t)b * F;
+
+  for (int i = tid; i < 2 * NBUF; i += blockDim.x) smem[i] = 0.0f;
+  for (int i = tid; i < a.nsteps; i += blockDim.x) xs[i] = a.x[(int64_t)b * a.T + a.n0 + i];
+  __syncthreads();
+
+  float U0[R], U1[R], KA[R], KB[R], KC[R], KD[R], KS[R];
+  const int row0 = g.k * R;  // region row of cell 0
+  uint32_t vmask = 0;        // cells that are inside the region
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    float u1 = 0.f, u2 = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f;
+    if (g.active && row0 + r < g.RH) {
+      vmask |= 1u << r;
+      const int64_t q = (int64_t)(g.R0 + row0 + r) * Ny + (g.C0 + g.c);
+      if (NL == 0) {
+        v0 = a.A[q]; v1 = a.Bm[q]; v2 = a.C2[q];
+      } else {
+        v0 = a.bgeom[q]; v1 = a.clin[q]; v2 = a.rho[q]; v3 = a.rhoc[q];
+      }
+      u1 = a.in1[boff + q];
+      u2 = a.in2[boff + q];
+    }
+    U0[r] = u1; U1[r] = u2;
+    KA[r] = v0; KB[r] = v1; KC[r] = v2; KD[r] = v3; KS[r] = 0.0f;
+    if (g.active) buf0[g.base + r] = u1;
+  }
+  // source pixels of this thread (multiplicity 1; the host routes pixels
+  // shared by several sources to the other kernels)
+  bool any_src = false;
+  for (int kk = td[8]; kk < td[9]; ++kk) {
+    const int32_t *e = a.spx + kk * SPX_W;
+    if (e[2] <= 0) continue;
+    const int rr = e[0] - g.R0 - row0, cc = e[1] - g.C0;
+    if (!g.active || cc != g.c || rr < 0 || rr >= R || row0 + rr >= g.RH) continue;
+    any_src = true;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (r == rr) KS[r] = 1.0f;
+  }
+  const bool blk_src = __syncthreads_or(any_src);
+  // probe tap handled by this thread (taps in the output region)
+  int pidx = -1, pkind = 0, pcol = 0;
+  {
+    int kk = 0;
+    for (int t = td[10]; t < td[11]; ++t) {
+      const int32_t *tp = a.taps + t * TAP_W;
+      if (tp[1] < g.oc0 || tp[1] >= g.oc1) continue;
+      if (kk == tid) {
+        pidx = (tp[1] - g.C0 + 1) * g.CS + (tp[0] - g.R0) + 1;
+        pkind = tp[2];
+        pcol = tp[3];
+      }
+      ++kk;
+    }
+  }
+  const uint32_t voff = cs_out_voff(g, row0, Ny);
+  const int64_t out_bytes = (int64_t)(g.or1 - g.or0) * Ny * 4;
+  __syncthreads();
+
+  auto step = [&](const float *P, float *Q, float(&Uc)[R], float(&Up)[R], int s) {
+    const int n = a.n0 + s;
+    int lo, hi;
+    cone(g, s, a.nsteps, lo, hi);
+    const bool need = g.active && row0 < hi && row0 + R > lo;
+    // the (sample, step) slice of the record, based at the first output row
+    const int64_t ro = ((int64_t)b * a.rec_K + (n - a.rec_n0)) * F + (int64_t)g.or0 * Ny;
+    const __amdgpu_buffer_rsrc_t rsl = cs_rsrc(REC ? a.rec_lap + ro : a.in1, REC ? out_bytes : 0);
+    const __amdgpu_buffer_rsrc_t rsu =
+        cs_rsrc((REC && NL) ? a.rec_u + ro : a.in1, (REC && NL) ? out_bytes : 0);
+    const float xv = blk_src ? xs[s] : 0.0f;
+    if (need)
+      cs_fwd_rows<R, NL, REC>(a, P, Q, Uc, Up, KA, KB, KC, KD, KS, g.base, g.CS, vmask, xv, rsl,
+                              rsu, voff, Ny);
+    __syncthreads();
+    if (pidx >= 0) {
+      const float v = Q[pidx];
+      const int64_t o = ((int64_t)b * a.T + n) * a.ntap + pcol;
+      if (a.y) a.y[o] = pkind ? v * v : v;
+      if (a.yraw) a.yraw[o] = v;
+    }
+    if (a.fields && ((n + 1) % a.fstride) == 0) {
+      const int64_t nf = a.T / a.fstride;
+      cs_snapshot(g, Q, a.fields + ((int64_t)b * nf + (n + 1) / a.fstride - 1) * F, Ny);
+    }
+  };
+
+  int s = 0;
+  for (; s + 1 < a.nsteps; s += 2) {
+    step(buf0, buf1, U0, U1, s);
+    step(buf1, buf0, U1, U0, s + 1);
+  }
+  if (s < a.nsteps) step(buf0, buf1, U0, U1, s);
+  // final state: after an even number of steps u_n is in U0, u_{n-1} in U1
+  const bool odd = (a.nsteps & 1) != 0;
+  if (g.active) {
+    const int gc = g.C0 + g.c;
+    if (gc >= g.oc0 && gc < g.oc1) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int gr = g.R0 + row0 + r;
+        if (row0 + r < g.RH && gr >= g.or0 && gr < g.or1) {
+          const int64_t q = boff + (int64_t)gr * Ny + gc;
+          a.out1[q] = odd ? U1[r] : U0[r];
+          a.out2[q] = odd ? U0[r] : U1[r];
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ adjoint
+
+// Probe-seed table: at most this many probe pixels per region.
+constexpr int CS_MAX_SEEDS = 16;
+
+// One reverse step of this thread's segment.  SP: the wave holds
+// probe/source pixels (sr0, sr1).  The record of lap(u_{m-1}) is read through
+// a range-checked buffer based at the tile's first output row (zero outside).
+template <int R, bool SP>
+__device__ __forceinline__ void cs_adj_rows(
+    const float *__restrict__ P, float *__restrict__ Q, float (&Lc)[R], float (&Ln)[R],
+    float (&WL)[R], const float (&KA1)[R], const float (&KA2)[R], const float (&KW)[R],
+    float (&G)[R], int base, int CS, __amdgpu_buffer_rsrc_t rss, uint32_t voff, int Ny,
+    uint32_t omask, int sr0, int sr1, float gs0, float gs1, int sl0, int sl1,
+    float *__restrict__ gsrc_row) {
+  const int lfb = base - CS, rtb = base + CS;
+  // gradient-accumulation loads first; consumed after the stencil
+  float SG[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) SG[r] = cs_bload(rss, voff + (uint32_t)(r * Ny * 4));
+  float upv = P[base - 1];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const float ct = WL[r];
+    const float dn = (r < R - 1) ? WL[r + 1] : P[base + R];
+    const float lf = P[lfb + r], rt = P[rtb + r];
+    float lapw = upv + lf;
+    lapw = __builtin_fmaf(-4.0f, ct, lapw);
+    lapw = lapw + rt;
+    lapw = lapw + dn;
+    float lam = __builtin_fmaf(KA1[r], Lc[r], lapw);
+    lam = __builtin_fmaf(-KA2[r], Ln[r], lam);
+    if (SP) {
+      if (r == sr0) lam += gs0;
+      if (r == sr1) lam += gs1;
+    }
+    upv = ct;
+    Ln[r] = lam;
+    const float wl = KW[r] * lam;
+    WL[r] = wl;
+    Q[base + r] = wl;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) G[r] = __builtin_fmaf(SG[r], Ln[r], G[r]);
+  if (SP) {
+    // lambda_m at source pixels of the output rows -> dJ/dx[b, m-1]
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r == sr0 && sl0 >= 0 && ((omask >> r) & 1u)) gsrc_row[sl0] = Ln[r];
+      if (r == sr1 && sl1 >= 0 && ((omask >> r) & 1u)) gsrc_row[sl1] = Ln[r];
+    }
+  }
+}
+
+// Linear cell.  LDS holds w*lambda.
+template <int NT, int R>
+__global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
+  extern __shared__ float smem[];
+  const int tid = threadIdx.x;
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = w / a.ntiles, ti = w - (w / a.ntiles) * a.ntiles;
+  const int32_t *td = a.tiles + ti * TILE_W;
+  const CsGeom g = cs_geom<R>(td, tid);
+  const int NBUF = (g.RW + 2) * g.CS;
+  float *buf0 = smem, *buf1 = smem + NBUF;
+  float *seeds = smem + 2 * NBUF;  // (CS_MAX_SEEDS, nsteps) probe seeds
+  __shared__ int seed_px[CS_MAX_SEEDS];
+  __shared__ int nseeds;
+  const int Ny = a.Ny;
+  const int64_t F = (int64_t)a.Nx * Ny;
+  const int64_t boff = (int64_t)b * F;
+  const float s2 = a.s2;
+
+  for (int i = tid; i < 2 * NBUF; i += blockDim.x) smem[i] = 0.0f;
+  if (tid == 0) {
+    // probe pixels of the region (rows R0 .. R0+RH), in table order
+    int n = 0;
+    for (int kk = td[8]; kk < td[9] && n < CS_MAX_SEEDS; ++kk) {
+      const int32_t *e = a.spx + kk * SPX_W;
+      if (e[5] > e[4]) seed_px[n++] = kk;
+    }
+    nseeds = n;
+  }
+  __syncthreads();
+  // seeds[k][s] = sum over taps of pixel k of dy/du * ybar at step m = m_hi - s
+  for (int i = tid; i < nseeds * a.nsteps; i += blockDim.x) {
+    const int k = i / a.nsteps, s = i - (i / a.nsteps) * a.nsteps;
+    const int m = a.m_hi - s;
+    const int32_t *e = a.spx + seed_px[k] * SPX_W;
+    float gs = 0.0f;
+    if (a.ybar) {
+      for (int t = e[4]; t < e[5]; ++t) {
+        const int32_t *tp = a.taps + t * TAP_W;
+        const int64_t o = ((int64_t)b * a.T + (m - 1)) * a.ntap + tp[3];
+        const float gy = a.ybar[o];
+        gs += tp[2] ? 2.0f * a.yraw[o] * gy : gy;
+      }
+    }
+    seeds[i] = gs;
+  }
+
+  float WL[R], L1[R], L2[R], KA1[R], KA2[R], KW[R], G[R];
+  const int row0 = g.k * R;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    float A = 0.f, Bm = 0.f, C2 = 0.f, u1 = 0.f, u2 = 0.f;
+    if (g.active && row0 + r < g.RH) {
+      const int64_t q = (int64_t)(g.R0 + row0 + r) * Ny + (g.C0 + g.c);
+      A = a.A[q]; Bm = a.Bm[q]; C2 = a.C2[q];
+      u1 = a.in1[boff + q];
+      u2 = a.in2[boff + q];
+    }
+    KA1[r] = A * s2; KA2[r] = A * Bm; KW[r] = A * C2;
+    L1[r] = u1; L2[r] = u2; G[r] = 0.0f;
+    WL[r] = KW[r] * u1;
+    if (g.active) buf0[g.base + r] = WL[r];
+  }
+  // special rows of this thread: probe taps (seed slot) / source pixels (slot)
+  int sr0 = -1, sr1 = -1, sd0 = -1, sd1 = -1, sl0 = -1, sl1 = -1;
+  {
+    int nseed = 0;
+    for (int kk = td[8]; kk < td[9]; ++kk) {
+      const int32_t *e = a.spx + kk * SPX_W;
+      const bool has_tap = e[5] > e[4];
+      const int seed_slot = has_tap ? nseed : -1;
+      if (has_tap) ++nseed;
+      const int rr = e[0] - g.R0 - row0;
+      if (!g.active || e[1] - g.C0 != g.c || rr < 0 || rr >= R || row0 + rr >= g.RH) continue;
+      const int src_slot = e[2] > 0 ? e[3] : -1;
+      if (sr0 < 0) { sr0 = rr; sd0 = seed_slot; sl0 = src_slot; }
+      else { sr1 = rr; sd1 = seed_slot; sl1 = src_slot; }
+    }
+  }
+  const bool wave_sp = __ballot(sr0 >= 0) != 0;
+  // output cells of this thread (gradient write-back, dJ/dx)
+  uint32_t omask = 0;
+  if (g.active && g.C0 + g.c >= g.oc0 && g.C0 + g.c < g.oc1) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int gr = g.R0 + row0 + r;
+      if (row0 + r < g.RH && gr >= g.or0 && gr < g.or1) omask |= 1u << r;
+    }
+  }
+  const uint32_t voff = cs_out_voff(g, row0, Ny);
+  const int64_t out_bytes = (int64_t)(g.or1 - g.or0) * Ny * 4;
+  __syncthreads();
+
+  auto step = [&](const float *P, float *Q, float(&Lc)[R], float(&Ln)[R], int s) {
+    const int m = a.m_hi - s;
+    int lo, hi;
+    cone(g, s, a.nsteps, lo, hi);
+    const bool need = g.active && row0 < hi && row0 + R > lo;
+    const __amdgpu_buffer_rsrc_t rss = cs_rsrc(
+        a.seg_lap + ((int64_t)b * a.seg_K + (m - 1 - a.seg_n0)) * F + (int64_t)g.or0 * Ny,
+        out_bytes);
+    float *gsrc_row = a.gsrc + ((int64_t)b * a.T + (m - 1)) * a.nsrcslot;
+    if (need) {
+      if (wave_sp) {
+        const float gs0 = sd0 >= 0 ? seeds[sd0 * a.nsteps + s] : 0.0f;
+        const float gs1 = sd1 >= 0 ? seeds[sd1 * a.nsteps + s] : 0.0f;
+        cs_adj_rows<R, true>(P, Q, Lc, Ln, WL, KA1, KA2, KW, G, g.base, g.CS, rss, voff, Ny,
+                             omask, sr0, sr1, gs0, gs1, sl0, sl1, gsrc_row);
+      } else {
+        cs_adj_rows<R, false>(P, Q, Lc, Ln, WL, KA1, KA2, KW, G, g.base, g.CS, rss, voff, Ny,
+                              omask, -1, -1, 0.f, 0.f, -1, -1, gsrc_row);
+      }
+    }
+    __syncthreads();
+  };
+
+  int s = 0;
+  for (; s + 1 < a.nsteps; s += 2) {
+    step(buf0, buf1, L1, L2, s);   // lambda_m -> L2; now L2 = lambda_m, L1 = lambda_{m+1}
+    step(buf1, buf0, L2, L1, s + 1);
+  }
+  if (s < a.nsteps) step(buf0, buf1, L1, L2, s);
+  const bool odd = (a.nsteps & 1) != 0;
+  if (g.active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if ((omask >> r) & 1u) {
+        const int64_t q = boff + (int64_t)(g.R0 + row0 + r) * Ny + (g.C0 + g.c);
+        a.out1[q] = odd ? L2[r] : L1[r];
+        a.out2[q] = odd ? L1[r] : L2[r];
+        a.gacc[q] += G[r];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+
+static int cs_max_dyn_lds() {
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0, r = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&r, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      r = 64 * 1024;
+    v = r;
+  }
+  return v;
+}
+
+// Allows a kernel the largest dynamic LDS the device leaves next to the
+// kernel's own static LDS.
+static hipError_t cs_allow_lds(const void *kfn) {
+  hipFuncAttributes at{};
+  hipError_t e = hipFuncGetAttributes(&at, kfn);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             cs_max_dyn_lds() - (int)at.sharedSizeBytes);
+}
+
+#define WT_CS_VARIANTS(X) \
+  X(1024, 4)              \
+  X(1024, 6)              \
+  X(1024, 8)              \
+  X(1024, 10)             \
+  X(1024, 12)             \
+  X(512, 16)              \
+  X(512, 20)              \
+  X(512, 24)
+
+bool cs_variant_exists(int nt, int r) {
+#define X(A, B) if (nt == A && r == B) return true;
+  WT_CS_VARIANTS(X)
+#undef X
+  return false;
+}
+
+// LDS bytes of a region with RH rows and RW columns for rows-per-thread R.
+size_t cs_shmem_bytes(int R, int RH, int RW) {
+  int nseg = (RH + R - 1) / R;
+  int CS = (nseg * R + 2) | 1;
+  // field double buffer + staged inputs (forward) or probe seeds (adjoint)
+  return ((size_t)2 * (RW + 2) * CS + (size_t)CS_MAX_SEEDS * CS_MAX_TBLOCK) * sizeof(float);
+}
+
+// threads a region needs
+int cs_threads(int R, int RH, int RW) {
+  int nseg = (RH + R - 1) / R;
+  int t = nseg * RW;
+  return (t + 63) / 64 * 64;
+}
+
+template <int NT, int R>
+static hipError_t cs_fwd_dispatch(int nl, bool rec, const FwdArgs &a, int nblocks,
+                                  int nthreads, size_t shmem, hipStream_t s) {
+#define FT(NL, REC)                                                                   \
+  do {                                                                                \
+    auto kfn = k_fwd_cs<NT, R, NL, REC>;                                              \
+    static bool done = false;                                                         \
+    if (!done) {                                                                      \
+      hipError_t e = cs_allow_lds((const void *)kfn);                                 \
+      if (e != hipSuccess) return e;                                                  \
+      done = true;                                                                    \
+    }                                                                                 \
+    hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, a);              \
+  } while (0)
+  if (nl == 0) {
+    if (rec) FT(0, true); else FT(0, false);
+  } else {
+    if (rec) FT(8, true); else FT(8, false);
+  }
+#undef FT
+  return hipGetLastError();
+}
+
+template <int NT, int R>
+static hipError_t cs_adj_dispatch(const AdjArgs &a, int nblocks, int nthreads, size_t shmem,
+                                  hipStream_t s) {
+  auto kfn = k_adj_cs<NT, R>;
+  static bool done = false;
+  if (!done) {
+    hipError_t e = cs_allow_lds((const void *)kfn);
+    if (e != hipSuccess) return e;
+    done = true;
+  }
+  hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, a);
+  return hipGetLastError();
+}
+
+// The region geometry (RH, RW) of the largest tile decides the block size
+// and LDS of the launch; smaller tiles leave threads idle.
+hipError_t launch_fwd_cs(int nt, int R, int nl, bool rec, const FwdArgs &a, int RHmax,
+                         int RWmax, hipStream_t s) {
+  int nblocks = a.B * a.ntiles;
+  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax);
+  int nthreads = cs_threads(R, RHmax, RWmax);
+  if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt) return hipErrorInvalidConfiguration;
+#define X(A, B) \
+  if (nt == A && R == B) return cs_fwd_dispatch<A, B>(nl, rec, a, nblocks, nthreads, shmem, s);
+  WT_CS_VARIANTS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj_cs(int nt, int R, const AdjArgs &a, int RHmax, int RWmax, hipStream_t s) {
+  int nblocks = a.B * a.ntiles;
+  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax);
+  int nthreads = cs_threads(R, RHmax, RWmax);
+  if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt) return hipErrorInvalidConfiguration;
+#define X(A, B) if (nt == A && R == B) return cs_adj_dispatch<A, B>(a, nblocks, nthreads, shmem, s);
+  WT_CS_VARIANTS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+}  // namespace wt

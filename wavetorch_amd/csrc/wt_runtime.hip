@@ -62,6 +62,12 @@ hipError_t launch_adj_tiled(int nt, int cpt, bool unit_h, const AdjArgs &a, int 
                             hipStream_t s);
 bool tiled_variant_exists(int nt, int cpt);
 size_t tiled_shmem_bytes(int nt, int cpt, int LW);
+bool cs_variant_exists(int nt, int r);
+size_t cs_shmem_bytes(int R, int RH, int RW);
+int cs_threads(int R, int RH, int RW);
+hipError_t launch_fwd_cs(int nt, int R, int nl, bool rec, const FwdArgs &a, int RHmax,
+                         int RWmax, hipStream_t s);
+hipError_t launch_adj_cs(int nt, int R, const AdjArgs &a, int RHmax, int RWmax, hipStream_t s);
 hipError_t launch_mul_scalar(const float *in, float v, int64_t n, float *out, hipStream_t s);
 hipError_t launch_clamp(const float *in, float lo, float hi, int64_t n, float *out,
                         hipStream_t s);
@@ -133,7 +139,7 @@ struct Plan {
   std::vector<int32_t> h_spx, h_taps, h_tiles, h_mult;
   int32_t *d_spx = nullptr, *d_taps = nullptr, *d_spmap = nullptr, *d_mult = nullptr;
   int32_t *d_tiles = nullptr;
-  int ntiles = 1, LWmax = 0;
+  int ntiles = 1, LWmax = 0, RHmax = 0, RWmax = 0;
   bool tiled = false;
   // workspace layout (bytes)
   size_t F = 0;
@@ -159,6 +165,8 @@ static void free_plan(Plan *p) {
 static void build_tiles(Plan &p, int strips, int cols, int halo) {
   p.h_tiles.clear();
   p.LWmax = 0;
+  p.RHmax = 0;
+  p.RWmax = 0;
   for (int si = 0; si < strips; ++si) {
     int o_r0 = (int)((int64_t)p.Nx * si / strips), o_r1 = (int)((int64_t)p.Nx * (si + 1) / strips);
     for (int ci = 0; ci < cols; ++ci) {
@@ -183,6 +191,8 @@ static void build_tiles(Plan &p, int strips, int cols, int halo) {
       t[10] = tl; t[11] = th;
       p.h_tiles.insert(p.h_tiles.end(), t, t + TILE_W);
       p.LWmax = std::max(p.LWmax, (C1 - C0) + 1);
+      p.RHmax = std::max(p.RHmax, R1 - R0);
+      p.RWmax = std::max(p.RWmax, C1 - C0);
     }
   }
   p.ntiles = strips * cols;
@@ -216,6 +226,51 @@ static bool tiles_ok(const Plan &p, int nt, int cpt) {
     if (ntp > nt) return false;
   }
   return true;
+}
+
+// Column-segment family: block of cs_threads() threads, R rows per thread;
+// per thread at most 2 special rows (adjoint) and source multiplicity <= 2.
+static bool tiles_ok_cs(const Plan &p, int nt, int R) {
+  if (!cs_variant_exists(nt, R)) return false;
+  if (cs_shmem_bytes(R, p.RHmax, p.RWmax) + 1024 > LDS_LIMIT) return false;
+  int nthr = cs_threads(R, p.RHmax, p.RWmax);
+  if (nthr > nt) return false;
+  for (int ti = 0; ti < p.ntiles; ++ti) {
+    const int32_t *t = &p.h_tiles[ti * TILE_W];
+    int RH = t[1], RW = t[3];
+    std::vector<int> spc((size_t)RW * ((RH + R - 1) / R), 0);
+    int nseed = 0;
+    for (int k = t[8]; k < t[9]; ++k) {
+      const int32_t *e = &p.h_spx[k * SPX_W];
+      if (e[5] > e[4] && ++nseed > 16) return false;
+      int rr = e[0] - t[0], cc = e[1] - t[2];
+      if (rr < 0 || rr >= RH || cc < 0 || cc >= RW) continue;
+      if (e[2] > 1) return false;
+      if (++spc[(size_t)(rr / R) * RW + cc] > 2) return false;
+    }
+    int ntp = 0;
+    for (int k = t[10]; k < t[11]; ++k) {
+      const int32_t *tp = &p.h_taps[k * TAP_W];
+      if (tp[1] >= t[6] && tp[1] < t[7]) ++ntp;
+    }
+    if (ntp > nthr) return false;
+  }
+  return true;
+}
+
+static const int kCsVariants[][2] = {{1024, 4},  {1024, 6},  {1024, 8},  {1024, 10},
+                                     {1024, 12}, {512, 16},  {512, 20},  {512, 24}};
+
+// Smallest R whose block fits (fewest rows per thread -> most threads).
+static bool pick_variant_cs(Plan &p, int &nt, int &R) {
+  for (auto &v : kCsVariants) {
+    if (tiles_ok_cs(p, v[0], v[1])) {
+      nt = v[0];
+      R = v[1];
+      return true;
+    }
+  }
+  return false;
 }
 
 static const int kVariants[][2] = {{1024, 8}, {1024, 12}, {1024, 16}, {512, 16},
@@ -463,23 +518,25 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
     int strips = tun.strips, tb = tun.tblock, nt = tun.nt, cpt = tun.cpt;
     if (tb <= 0) tb = 16;
     if (tb > p->K) tb = p->K;
+    if (tun.kind == 0 && tb > 256) tb = 256;  // inputs/seeds of a launch are staged in LDS
     if (strips <= 0) {
       // heuristic: enough workgroups to cover the 256 CUs, region fitting
       // the largest variant
       strips = 1;
       while (strips < p->Nx / 4 && (int64_t)B * strips < 256) ++strips;
     }
+    int tcols = (tun.kind == 0 && tun.tcols > 0) ? tun.tcols : 1;
     bool ok = false;
     for (; strips <= p->Nx; ++strips) {
       if (strips > 1 && tb > (p->Nx / strips)) {
         // a halo deeper than the strip is allowed but wasteful; keep going
       }
-      build_tiles(*p, strips, 1, tb);
+      build_tiles(*p, strips, tcols, tb);
       if (nt > 0 && cpt > 0) {
-        ok = tiles_ok(*p, nt, cpt);
+        ok = tun.kind == 1 ? tiles_ok(*p, nt, cpt) : tiles_ok_cs(*p, nt, cpt);
         if (!ok && tun.strips > 0) break;
       } else {
-        ok = pick_variant(*p, nt, cpt);
+        ok = tun.kind == 1 ? pick_variant(*p, nt, cpt) : pick_variant_cs(*p, nt, cpt);
       }
       if (ok) break;
       if (tun.strips > 0) break;
@@ -494,6 +551,7 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
       tiled = false;
     } else {
       tun.strips = strips;
+      tun.tcols = tcols;
       tun.tblock = tb;
       tun.nt = nt;
       tun.cpt = cpt;
@@ -501,6 +559,7 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
   }
   if (!tiled) {
     tun.plain = 1;
+    tun.tcols = 0;
     tun.strips = 0;
     tun.tblock = 1;
     tun.nt = 256;
@@ -644,7 +703,10 @@ int run_forward(const Plan &p, FwdArgs a, int n0, int n1, float *bufs[6], float 
       a.in1 = i1; a.in2 = i2; a.out1 = o1; a.out2 = o2;
       a.n0 = n; a.nsteps = k;
       ProfScope ps(rec ? 1 : 0, s, (int64_t)p.B * p.Nx * p.Ny * k);
-      WT_HIP(launch_fwd_tiled(p.tun.nt, p.tun.cpt, p.unit_h, p.nl, rec, a, p.LWmax, s));
+      if (p.tun.kind == 1)
+        WT_HIP(launch_fwd_tiled(p.tun.nt, p.tun.cpt, p.unit_h, p.nl, rec, a, p.LWmax, s));
+      else
+        WT_HIP(launch_fwd_cs(p.tun.nt, p.tun.cpt, p.nl, rec, a, p.RHmax, p.RWmax, s));
       std::swap(i1, o1);
       std::swap(i2, o2);
       n += k;
@@ -796,7 +858,10 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
         aa.in1 = l1; aa.in2 = l2; aa.out1 = l3; aa.out2 = l4;
         aa.m_hi = m; aa.nsteps = k;
         ProfScope ps(2, s, (int64_t)p->B * p->Nx * p->Ny * k);
-        WT_HIP(launch_adj_tiled(p->tun.nt, p->tun.cpt, p->unit_h, aa, p->LWmax, s));
+        if (p->tun.kind == 1)
+          WT_HIP(launch_adj_tiled(p->tun.nt, p->tun.cpt, p->unit_h, aa, p->LWmax, s));
+        else
+          WT_HIP(launch_adj_cs(p->tun.nt, p->tun.cpt, aa, p->RHmax, p->RWmax, s));
         std::swap(l1, l3);
         std::swap(l2, l4);
         m -= k;
