@@ -122,6 +122,8 @@ typedef struct wt_tuning {
   int32_t kind;      /* tiled kernel family: 0 = column segments (cpt = rows
                         per thread), 1 = flat cell list */
   int32_t tcols;     /* column tiles per sample (column-segment family; 0 = 1) */
+  /* adjoint kernel choice; 0 = same as the forward fields above */
+  int32_t a_strips, a_tcols, a_tblock, a_nt, a_cpt;
 } wt_tuning;
 
 /* Creates a plan for `cfg` at batch B and length T: validates the

@@ -33,6 +33,8 @@ TUNINGS = [
     {"kind": 0, "strips": 6, "tblock": 3, "nt": 512, "cpt": 16},
     {"kind": 0, "strips": 2, "tcols": 2, "tblock": 8, "nt": 1024, "cpt": 8},
     {"kind": 0, "strips": 3, "tcols": 3, "tblock": 6, "nt": 1024, "cpt": 4},
+    {"kind": 0, "strips": 2, "tcols": 2, "tblock": 12, "nt": 1024, "cpt": 8,
+     "a_tblock": 10, "a_nt": 512, "a_cpt": 16},
 ]
 
 
@@ -92,7 +94,9 @@ def test_train_shape_bitwise_and_grad(cuda):
     p = dict(b=b, c=c, x=x, src=[[40, 75, 0]], taps=[[110, 55, 1], [110, 75, 1], [110, 95, 1]],
              dt=0.707, h=1.0, Nx=N, Ny=N, B=B, T=T)
     for tun in ({}, {"kind": 1}, {"plain": 1},
-                {"strips": 2, "tcols": 2, "tblock": 12, "nt": 1024, "cpt": 8}):
+                {"strips": 2, "tcols": 2, "tblock": 12, "nt": 1024, "cpt": 8},
+                {"strips": 2, "tcols": 2, "tblock": 12, "nt": 1024, "cpt": 8,
+                 "a_tblock": 10, "a_nt": 512, "a_cpt": 16}):
         print("case", tun, flush=True)
         plan, xt, ct, y = _run(p, tun, cuda, grad=True, K=64)
         yo, _ = co.forward(b, c, x, 0.707, 1.0, p["src"], p["taps"])

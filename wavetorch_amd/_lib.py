@@ -11,7 +11,8 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libwavetorch_hip.so")
+LIB_PATH = os.environ.get("WAVETORCH_AMD_LIB",
+                          os.path.join(_HERE, "_lib", "libwavetorch_hip.so"))
 
 WT_OK = 0
 WT_EINVAL = -1
@@ -42,7 +43,8 @@ class WtTuning(ctypes.Structure):
     """Mirror of ``wt_tuning``."""
     _fields_ = [("plain", c_i32), ("strips", c_i32), ("tblock", c_i32), ("nt", c_i32),
                 ("cpt", c_i32), ("ck_every", c_i32), ("kind", c_i32),
-                ("tcols", c_i32)]
+                ("tcols", c_i32), ("a_strips", c_i32), ("a_tcols", c_i32), ("a_tblock", c_i32),
+                ("a_nt", c_i32), ("a_cpt", c_i32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -33,8 +33,8 @@ FLAGS = [
 ]
 
 
-def _compile(src: str, obj: str, verbose: bool) -> None:
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(HERE, src), "-o", obj]
+def _compile(src: str, obj: str, verbose: bool, extra=()) -> None:
+    cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(HERE, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -46,17 +46,21 @@ def _compile(src: str, obj: str, verbose: bool) -> None:
         print(msg)
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
-    """Compiles every kernel source and links the shared library; returns its path."""
-    os.makedirs(OUT, exist_ok=True)
-    objdir = os.path.join(OUT, "obj")
+def build(verbose: bool = False, jobs: int = 4, out_dir: str | None = None,
+          extra_flags=()) -> str:
+    """Compiles every kernel source and links the shared library; returns its path.
+    ``out_dir``/``extra_flags`` build a variant elsewhere (diagnostic builds)."""
+    out = out_dir or OUT
+    os.makedirs(out, exist_ok=True)
+    objdir = os.path.join(out, "obj")
     os.makedirs(objdir, exist_ok=True)
     objs = [os.path.join(objdir, s.replace(".hip", ".o")) for s in SOURCES]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(_compile, s, o, verbose) for s, o in zip(SOURCES, objs)]
+        futs = [ex.submit(_compile, s, o, verbose, tuple(extra_flags))
+                for s, o in zip(SOURCES, objs)]
         for f in futs:
             f.result()
-    lib = os.path.join(OUT, LIBNAME)
+    lib = os.path.join(out, LIBNAME)
     tmp = lib + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

@@ -35,6 +35,8 @@ namespace wt {
 // Largest number of steps one launch of these kernels may take (the input
 // samples of a launch are staged in LDS).
 constexpr int CS_MAX_TBLOCK = 256;
+// Largest number of special pixels / probe taps one tile may hold.
+constexpr int CS_MAX_TABLE = 256;
 
 struct CsGeom {
   int R0, RH, C0, RW;     // region
@@ -167,8 +169,13 @@ __device__ __forceinline__ void cs_fwd_rows(const FwdArgs &a, const float *__res
   }
 }
 
+#ifdef WT_PHASE_TIMING
+__device__ unsigned long long *wt_phase_ts = nullptr;
+#endif
+
 template <int NT, int R, int NL, bool REC>
 __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
+  WT_TS(0);
   extern __shared__ float smem[];
   const int tid = threadIdx.x;
   const int w = xcd_remap(blockIdx.x, gridDim.x);
@@ -182,13 +189,16 @@ __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
   const int64_t F = (int64_t)a.Nx * Ny;
   const int64_t boff = (int64_t)b * F;
 
-  for (int i = tid; i < 2 * NBUF; i += blockDim.x) smem[i] = 0.0f;
-  for (int i = tid; i < a.nsteps; i += blockDim.x) xs[i] = a.x[(int64_t)b * a.T + a.n0 + i];
-  __syncthreads();
+  // tables of this tile staged in LDS (one parallel load, then scans hit
+  // LDS): special pixels (x, y, multiplicity) and probe taps (x, y, kind, col)
+  __shared__ int tb_sp[CS_MAX_TABLE * 3];
+  __shared__ int tb_tp[CS_MAX_TABLE * 4];
+  const int nsp = min(td[9] - td[8], CS_MAX_TABLE), ntp = min(td[11] - td[10], CS_MAX_TABLE);
 
   float U0[R], U1[R], KA[R], KB[R], KC[R], KD[R], KS[R];
   const int row0 = g.k * R;  // region row of cell 0
   uint32_t vmask = 0;        // cells that are inside the region
+  // 1) all global loads first (one round trip for the whole set-up)
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     float u1 = 0.f, u2 = 0.f, v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f;
@@ -205,15 +215,39 @@ __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
     }
     U0[r] = u1; U1[r] = u2;
     KA[r] = v0; KB[r] = v1; KC[r] = v2; KD[r] = v3; KS[r] = 0.0f;
-    if (g.active) buf0[g.base + r] = u1;
+  }
+  float xsv = 0.0f;
+  if (tid < a.nsteps) xsv = a.x[(int64_t)b * a.T + a.n0 + tid];
+  int spv0 = 0, spv1 = 0, spv2 = 0, tpv0 = 0, tpv1 = 0, tpv2 = 0, tpv3 = 0;
+  if (tid < nsp) {
+    const int32_t *e = a.spx + (td[8] + tid) * SPX_W;
+    spv0 = e[0]; spv1 = e[1]; spv2 = e[2];
+  }
+  if (tid < ntp) {
+    const int32_t *t = a.taps + (td[10] + tid) * TAP_W;
+    tpv0 = t[0]; tpv1 = t[1]; tpv2 = t[2]; tpv3 = t[3];
+  }
+  // 2) LDS
+  for (int i = tid; i < 2 * NBUF; i += blockDim.x) smem[i] = 0.0f;
+  WT_TS(4);
+  if (tid < a.nsteps) xs[tid] = xsv;
+  if (tid < nsp) { tb_sp[3 * tid] = spv0; tb_sp[3 * tid + 1] = spv1; tb_sp[3 * tid + 2] = spv2; }
+  if (tid < ntp) {
+    tb_tp[4 * tid] = tpv0; tb_tp[4 * tid + 1] = tpv1;
+    tb_tp[4 * tid + 2] = tpv2; tb_tp[4 * tid + 3] = tpv3;
+  }
+  __syncthreads();
+  WT_TS(5);
+  if (g.active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf0[g.base + r] = U0[r];
   }
   // source pixels of this thread (multiplicity 1; the host routes pixels
   // shared by several sources to the other kernels)
   bool any_src = false;
-  for (int kk = td[8]; kk < td[9]; ++kk) {
-    const int32_t *e = a.spx + kk * SPX_W;
-    if (e[2] <= 0) continue;
-    const int rr = e[0] - g.R0 - row0, cc = e[1] - g.C0;
+  for (int kk = 0; kk < nsp; ++kk) {
+    if (tb_sp[3 * kk + 2] <= 0) continue;
+    const int rr = tb_sp[3 * kk] - g.R0 - row0, cc = tb_sp[3 * kk + 1] - g.C0;
     if (!g.active || cc != g.c || rr < 0 || rr >= R || row0 + rr >= g.RH) continue;
     any_src = true;
 #pragma unroll
@@ -221,17 +255,18 @@ __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
       if (r == rr) KS[r] = 1.0f;
   }
   const bool blk_src = __syncthreads_or(any_src);
+  WT_TS(6);
   // probe tap handled by this thread (taps in the output region)
   int pidx = -1, pkind = 0, pcol = 0;
   {
     int kk = 0;
-    for (int t = td[10]; t < td[11]; ++t) {
-      const int32_t *tp = a.taps + t * TAP_W;
-      if (tp[1] < g.oc0 || tp[1] >= g.oc1) continue;
+    for (int t = 0; t < ntp; ++t) {
+      const int tx = tb_tp[4 * t], ty = tb_tp[4 * t + 1];
+      if (ty < g.oc0 || ty >= g.oc1) continue;
       if (kk == tid) {
-        pidx = (tp[1] - g.C0 + 1) * g.CS + (tp[0] - g.R0) + 1;
-        pkind = tp[2];
-        pcol = tp[3];
+        pidx = (ty - g.C0 + 1) * g.CS + (tx - g.R0) + 1;
+        pkind = tb_tp[4 * t + 2];
+        pcol = tb_tp[4 * t + 3];
       }
       ++kk;
     }
@@ -267,12 +302,14 @@ __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
     }
   };
 
+  WT_TS(1);
   int s = 0;
   for (; s + 1 < a.nsteps; s += 2) {
     step(buf0, buf1, U0, U1, s);
     step(buf1, buf0, U1, U0, s + 1);
   }
   if (s < a.nsteps) step(buf0, buf1, U0, U1, s);
+  WT_TS(2);
   // final state: after an even number of steps u_n is in U0, u_{n-1} in U1
   const bool odd = (a.nsteps & 1) != 0;
   if (g.active) {
@@ -289,6 +326,7 @@ __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
       }
     }
   }
+  WT_TS(3);
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -297,20 +335,24 @@ __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
 constexpr int CS_MAX_SEEDS = 16;
 
 // One reverse step of this thread's segment.  SP: the wave holds
-// probe/source pixels (sr0, sr1).  The record of lap(u_{m-1}) is read through
-// a range-checked buffer based at the tile's first output row (zero outside).
+// probe/source pixels (sr0, sr1).  SGc holds lap(u_{m-1}) of this step
+// (prefetched one step earlier); the loads of the next step's values go
+// into SGn first, so their latency is covered by a whole step.  The record
+// is read through a range-checked buffer based at the tile's first output
+// row (zero outside the output rows).  a2 = A*Bm is formed as a1 - 1 (with
+// a1 = A*s2 = 2/(1+b*dt), a2 = (1-b*dt)/(1+b*dt)).
 template <int R, bool SP>
 __device__ __forceinline__ void cs_adj_rows(
     const float *__restrict__ P, float *__restrict__ Q, float (&Lc)[R], float (&Ln)[R],
-    float (&WL)[R], const float (&KA1)[R], const float (&KA2)[R], const float (&KW)[R],
-    float (&G)[R], int base, int CS, __amdgpu_buffer_rsrc_t rss, uint32_t voff, int Ny,
-    uint32_t omask, int sr0, int sr1, float gs0, float gs1, int sl0, int sl1,
-    float *__restrict__ gsrc_row) {
+    float (&WL)[R], const float (&KA1)[R], const float (&KW)[R], float (&G)[R],
+    const float (&SGc)[R], float (&SGn)[R], int base, int CS, __amdgpu_buffer_rsrc_t rsn,
+    bool prefetch, uint32_t voff, int Ny, uint32_t omask, int sr0, int sr1, float gs0,
+    float gs1, int sl0, int sl1, float *__restrict__ gsrc_row) {
   const int lfb = base - CS, rtb = base + CS;
-  // gradient-accumulation loads first; consumed after the stencil
-  float SG[R];
+  if (prefetch) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) SG[r] = cs_bload(rss, voff + (uint32_t)(r * Ny * 4));
+    for (int r = 0; r < R; ++r) SGn[r] = cs_bload(rsn, voff + (uint32_t)(r * Ny * 4));
+  }
   float upv = P[base - 1];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -321,8 +363,8 @@ __device__ __forceinline__ void cs_adj_rows(
     lapw = __builtin_fmaf(-4.0f, ct, lapw);
     lapw = lapw + rt;
     lapw = lapw + dn;
-    float lam = __builtin_fmaf(KA1[r], Lc[r], lapw);
-    lam = __builtin_fmaf(-KA2[r], Ln[r], lam);
+    // a1*L1 + lapw - (a1-1)*L2 = a1*(L1 - L2) + (lapw + L2)
+    float lam = __builtin_fmaf(KA1[r], Lc[r] - Ln[r], lapw + Ln[r]);
     if (SP) {
       if (r == sr0) lam += gs0;
       if (r == sr1) lam += gs1;
@@ -332,9 +374,8 @@ __device__ __forceinline__ void cs_adj_rows(
     const float wl = KW[r] * lam;
     WL[r] = wl;
     Q[base + r] = wl;
+    G[r] = __builtin_fmaf(SGc[r], lam, G[r]);
   }
-#pragma unroll
-  for (int r = 0; r < R; ++r) G[r] = __builtin_fmaf(SG[r], Ln[r], G[r]);
   if (SP) {
     // lambda_m at source pixels of the output rows -> dJ/dx[b, m-1]
 #pragma unroll
@@ -348,6 +389,7 @@ __device__ __forceinline__ void cs_adj_rows(
 // Linear cell.  LDS holds w*lambda.
 template <int NT, int R>
 __global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
+  WT_TS(0);
   extern __shared__ float smem[];
   const int tid = threadIdx.x;
   const int w = xcd_remap(blockIdx.x, gridDim.x);
@@ -364,22 +406,61 @@ __global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
   const int64_t boff = (int64_t)b * F;
   const float s2 = a.s2;
 
-  for (int i = tid; i < 2 * NBUF; i += blockDim.x) smem[i] = 0.0f;
-  if (tid == 0) {
-    // probe pixels of the region (rows R0 .. R0+RH), in table order
-    int n = 0;
-    for (int kk = td[8]; kk < td[9] && n < CS_MAX_SEEDS; ++kk) {
-      const int32_t *e = a.spx + kk * SPX_W;
-      if (e[5] > e[4]) seed_px[n++] = kk;
+  __shared__ int tb_sp[CS_MAX_TABLE * 6];  // x, y, mult, slot, tap_lo, tap_hi
+  const int nsp = min(td[9] - td[8], CS_MAX_TABLE);
+
+  float WL[R], L1[R], L2[R], KA1[R], KW[R], G[R], SG0[R], SG1[R];
+  const int row0 = g.k * R;
+  // 1) all global loads first
+  float KC2[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    float A = 0.f, C2 = 0.f, u1 = 0.f, u2 = 0.f;
+    if (g.active && row0 + r < g.RH) {
+      const int64_t q = (int64_t)(g.R0 + row0 + r) * Ny + (g.C0 + g.c);
+      A = a.A[q]; C2 = a.C2[q];
+      u1 = a.in1[boff + q];
+      u2 = a.in2[boff + q];
     }
+    KA1[r] = A; KC2[r] = C2;
+    L1[r] = u1; L2[r] = u2;
+  }
+  int e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0;
+  if (tid < nsp) {
+    const int32_t *e = a.spx + (td[8] + tid) * SPX_W;
+    e0 = e[0]; e1 = e[1]; e2 = e[2]; e3 = e[3]; e4 = e[4]; e5 = e[5];
+  }
+  // 2) LDS
+  for (int i = tid; i < 2 * NBUF; i += blockDim.x) smem[i] = 0.0f;
+  if (tid < nsp) {
+    int *t = tb_sp + 6 * tid;
+    t[0] = e0; t[1] = e1; t[2] = e2; t[3] = e3; t[4] = e4; t[5] = e5;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const float A = KA1[r];
+    KA1[r] = A * s2; KW[r] = A * KC2[r];
+    G[r] = 0.0f; SG1[r] = 0.0f;
+    WL[r] = KW[r] * L1[r];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // probe pixels of the region, in table order
+    int n = 0;
+    for (int kk = 0; kk < nsp && n < CS_MAX_SEEDS; ++kk)
+      if (tb_sp[6 * kk + 5] > tb_sp[6 * kk + 4]) seed_px[n++] = kk;
     nseeds = n;
+  }
+  if (g.active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf0[g.base + r] = WL[r];
   }
   __syncthreads();
   // seeds[k][s] = sum over taps of pixel k of dy/du * ybar at step m = m_hi - s
   for (int i = tid; i < nseeds * a.nsteps; i += blockDim.x) {
     const int k = i / a.nsteps, s = i - (i / a.nsteps) * a.nsteps;
     const int m = a.m_hi - s;
-    const int32_t *e = a.spx + seed_px[k] * SPX_W;
+    const int *e = tb_sp + 6 * seed_px[k];
     float gs = 0.0f;
     if (a.ybar) {
       for (int t = e[4]; t < e[5]; ++t) {
@@ -391,29 +472,12 @@ __global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
     }
     seeds[i] = gs;
   }
-
-  float WL[R], L1[R], L2[R], KA1[R], KA2[R], KW[R], G[R];
-  const int row0 = g.k * R;
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    float A = 0.f, Bm = 0.f, C2 = 0.f, u1 = 0.f, u2 = 0.f;
-    if (g.active && row0 + r < g.RH) {
-      const int64_t q = (int64_t)(g.R0 + row0 + r) * Ny + (g.C0 + g.c);
-      A = a.A[q]; Bm = a.Bm[q]; C2 = a.C2[q];
-      u1 = a.in1[boff + q];
-      u2 = a.in2[boff + q];
-    }
-    KA1[r] = A * s2; KA2[r] = A * Bm; KW[r] = A * C2;
-    L1[r] = u1; L2[r] = u2; G[r] = 0.0f;
-    WL[r] = KW[r] * u1;
-    if (g.active) buf0[g.base + r] = WL[r];
-  }
   // special rows of this thread: probe taps (seed slot) / source pixels (slot)
   int sr0 = -1, sr1 = -1, sd0 = -1, sd1 = -1, sl0 = -1, sl1 = -1;
   {
     int nseed = 0;
-    for (int kk = td[8]; kk < td[9]; ++kk) {
-      const int32_t *e = a.spx + kk * SPX_W;
+    for (int kk = 0; kk < nsp; ++kk) {
+      const int *e = tb_sp + 6 * kk;
       const bool has_tap = e[5] > e[4];
       const int seed_slot = has_tap ? nseed : -1;
       if (has_tap) ++nseed;
@@ -438,47 +502,67 @@ __global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
   const int64_t out_bytes = (int64_t)(g.or1 - g.or0) * Ny * 4;
   __syncthreads();
 
-  auto step = [&](const float *P, float *Q, float(&Lc)[R], float(&Ln)[R], int s) {
+  auto seg_rsrc = [&](int m) {
+    return cs_rsrc(
+        a.seg_lap + ((int64_t)b * a.seg_K + (m - 1 - a.seg_n0)) * F + (int64_t)g.or0 * Ny,
+        out_bytes);
+  };
+  // lap(u_{m-1}) of the first step of this launch
+  {
+    const __amdgpu_buffer_rsrc_t rs0 = seg_rsrc(a.m_hi);
+#pragma unroll
+    for (int r = 0; r < R; ++r) SG0[r] = cs_bload(rs0, voff + (uint32_t)(r * Ny * 4));
+  }
+
+  auto step = [&](const float *P, float *Q, float(&Lc)[R], float(&Ln)[R], const float(&SGc)[R],
+                  float(&SGn)[R], int s) {
     const int m = a.m_hi - s;
     int lo, hi;
     cone(g, s, a.nsteps, lo, hi);
     const bool need = g.active && row0 < hi && row0 + R > lo;
-    const __amdgpu_buffer_rsrc_t rss = cs_rsrc(
-        a.seg_lap + ((int64_t)b * a.seg_K + (m - 1 - a.seg_n0)) * F + (int64_t)g.or0 * Ny,
-        out_bytes);
+    const bool prefetch = s + 1 < a.nsteps;
+    const __amdgpu_buffer_rsrc_t rsn = seg_rsrc(prefetch ? m - 1 : m);
     float *gsrc_row = a.gsrc + ((int64_t)b * a.T + (m - 1)) * a.nsrcslot;
     if (need) {
       if (wave_sp) {
         const float gs0 = sd0 >= 0 ? seeds[sd0 * a.nsteps + s] : 0.0f;
         const float gs1 = sd1 >= 0 ? seeds[sd1 * a.nsteps + s] : 0.0f;
-        cs_adj_rows<R, true>(P, Q, Lc, Ln, WL, KA1, KA2, KW, G, g.base, g.CS, rss, voff, Ny,
-                             omask, sr0, sr1, gs0, gs1, sl0, sl1, gsrc_row);
+        cs_adj_rows<R, true>(P, Q, Lc, Ln, WL, KA1, KW, G, SGc, SGn, g.base, g.CS, rsn, prefetch,
+                             voff, Ny, omask, sr0, sr1, gs0, gs1, sl0, sl1, gsrc_row);
       } else {
-        cs_adj_rows<R, false>(P, Q, Lc, Ln, WL, KA1, KA2, KW, G, g.base, g.CS, rss, voff, Ny,
-                              omask, -1, -1, 0.f, 0.f, -1, -1, gsrc_row);
+        cs_adj_rows<R, false>(P, Q, Lc, Ln, WL, KA1, KW, G, SGc, SGn, g.base, g.CS, rsn,
+                              prefetch, voff, Ny, omask, -1, -1, 0.f, 0.f, -1, -1, gsrc_row);
       }
     }
     __syncthreads();
   };
 
+  WT_TS(1);
   int s = 0;
   for (; s + 1 < a.nsteps; s += 2) {
-    step(buf0, buf1, L1, L2, s);   // lambda_m -> L2; now L2 = lambda_m, L1 = lambda_{m+1}
-    step(buf1, buf0, L2, L1, s + 1);
+    step(buf0, buf1, L1, L2, SG0, SG1, s);   // lambda_m -> L2
+    step(buf1, buf0, L2, L1, SG1, SG0, s + 1);
   }
-  if (s < a.nsteps) step(buf0, buf1, L1, L2, s);
+  if (s < a.nsteps) step(buf0, buf1, L1, L2, SG0, SG1, s);
+  WT_TS(2);
   const bool odd = (a.nsteps & 1) != 0;
   if (g.active) {
+    // running gradient sums: all loads first, then the stores
+    const int64_t q0 = boff + (int64_t)(g.R0 + row0) * Ny + (g.C0 + g.c);
+    float gold[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) gold[r] = ((omask >> r) & 1u) ? a.gacc[q0 + (int64_t)r * Ny] : 0.0f;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if ((omask >> r) & 1u) {
-        const int64_t q = boff + (int64_t)(g.R0 + row0 + r) * Ny + (g.C0 + g.c);
+        const int64_t q = q0 + (int64_t)r * Ny;
         a.out1[q] = odd ? L2[r] : L1[r];
         a.out2[q] = odd ? L1[r] : L2[r];
-        a.gacc[q] += G[r];
+        a.gacc[q] = gold[r] + G[r];
       }
     }
   }
+  WT_TS(3);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -514,6 +598,12 @@ static hipError_t cs_allow_lds(const void *kfn) {
   X(512, 16)              \
   X(512, 20)              \
   X(512, 24)
+
+#ifdef WT_PHASE_TIMING
+extern "C" int wt_debug_phase_buffer(unsigned long long *p) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(wt_phase_ts), &p, sizeof(p)) == hipSuccess ? 0 : -3;
+}
+#endif
 
 bool cs_variant_exists(int nt, int r) {
 #define X(A, B) if (nt == A && r == B) return true;

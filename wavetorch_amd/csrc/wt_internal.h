@@ -86,6 +86,23 @@ struct AdjArgs {
   float *gsrc;       // (B,T,nsrcslot) lambda_m at source pixels
 };
 
+// Optional phase timestamps (diagnostic builds compiled with
+// -DWT_PHASE_TIMING): thread 0 of every workgroup records the 100 MHz wall
+// clock at kernel start, after set-up, after the step loop and at the end
+// into phase_ts[(blockIdx.x) * 8 + i].
+#ifdef WT_PHASE_TIMING
+extern __device__ unsigned long long *wt_phase_ts;
+#define WT_TS(i)                                                             \
+  do {                                                                       \
+    if (wt_phase_ts && threadIdx.x == 0)                                     \
+      wt_phase_ts[blockIdx.x * 8 + (i)] = __builtin_amdgcn_s_memrealtime();  \
+  } while (0)
+#else
+#define WT_TS(i) \
+  do {           \
+  } while (0)
+#endif
+
 // XCD-aware work-item order.  Workgroups are dispatched round-robin over the
 // 8 XCDs of the chip (workgroup i runs on XCD i % 8); remap so that
 // consecutive work items (the strips of one sample, which exchange halos

@@ -140,6 +140,10 @@ struct Plan {
   int32_t *d_spx = nullptr, *d_taps = nullptr, *d_spmap = nullptr, *d_mult = nullptr;
   int32_t *d_tiles = nullptr;
   int ntiles = 1, LWmax = 0, RHmax = 0, RWmax = 0;
+  // adjoint tiling (column-segment family), may differ from the forward's
+  std::vector<int32_t> h_tiles_a;
+  int32_t *d_tiles_a = nullptr;
+  int ntiles_a = 1, RHmax_a = 0, RWmax_a = 0, tb_a = 1, nt_a = 0, cpt_a = 0;
   bool tiled = false;
   // workspace layout (bytes)
   size_t F = 0;
@@ -158,6 +162,7 @@ static void free_plan(Plan *p) {
   if (p->d_spmap) (void)hipFree(p->d_spmap);
   if (p->d_mult) (void)hipFree(p->d_mult);
   if (p->d_tiles) (void)hipFree(p->d_tiles);
+  if (p->d_tiles_a) (void)hipFree(p->d_tiles_a);
   delete p;
 }
 
@@ -232,6 +237,10 @@ static bool tiles_ok(const Plan &p, int nt, int cpt) {
 // per thread at most 2 special rows (adjoint) and source multiplicity <= 2.
 static bool tiles_ok_cs(const Plan &p, int nt, int R) {
   if (!cs_variant_exists(nt, R)) return false;
+  for (int ti = 0; ti < p.ntiles; ++ti) {  // tables staged in LDS per tile
+    const int32_t *t = &p.h_tiles[ti * TILE_W];
+    if (t[9] - t[8] > 256 || t[11] - t[10] > 256) return false;
+  }
   if (cs_shmem_bytes(R, p.RHmax, p.RWmax) + 1024 > LDS_LIMIT) return false;
   int nthr = cs_threads(R, p.RHmax, p.RWmax);
   if (nthr > nt) return false;
@@ -307,7 +316,7 @@ static int plan_layout(Plan &p) {
   p.saved_yraw = align_up((size_t)p.B * p.T * std::max(p.ntap, 1) * sizeof(float));
   p.saved_bytes = p.saved_coef + p.saved_ck + p.saved_yraw;
   // temp
-  p.t_state = align_up(6 * BF);  // tiled: 2x2 ping-pong; plain: 3 rotating (+ spare)
+  p.t_state = align_up(8 * BF);  // 4 state pairs (ping-pong + segment in/out)
   p.t_lam = align_up(4 * BF);    // adjoint state ping-pong (plain uses 3)
   p.t_wl = align_up(2 * BF);     // plain adjoint w*lambda
   p.t_gacc = align_up((nlc ? 3 : 1) * BF);
@@ -566,6 +575,50 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
     tun.cpt = 1;
   }
   tun.ck_every = p->K;
+  if (tiled && tun.kind == 0) {
+    // adjoint tiling: defaults to the forward's
+    std::vector<int32_t> ft = p->h_tiles;  // forward tiles
+    int fnt = p->ntiles, fRH = p->RHmax, fRW = p->RWmax, fLW = p->LWmax;
+    int as = tun.a_strips > 0 ? tun.a_strips : tun.strips;
+    int ac = tun.a_tcols > 0 ? tun.a_tcols : (tun.tcols > 0 ? tun.tcols : 1);
+    int atb = tun.a_tblock > 0 ? tun.a_tblock : tun.tblock;
+    if (atb > p->K) atb = p->K;
+    if (atb > 256) atb = 256;
+    int ant = tun.a_nt > 0 ? tun.a_nt : tun.nt;
+    int acp = tun.a_cpt > 0 ? tun.a_cpt : tun.cpt;
+    build_tiles(*p, as, ac, atb);
+    bool aok = tiles_ok_cs(*p, ant, acp);
+    if (!aok && tun.a_nt <= 0) aok = pick_variant_cs(*p, ant, acp);
+    if (!aok) {
+      set_error("wt_plan_create: adjoint tuning (strips=%d tcols=%d tblock=%d nt=%d cpt=%d) "
+                "does not fit", as, ac, atb, ant, acp);
+      free_plan(p);
+      return WT_EUNSUPPORTED;
+    }
+    p->h_tiles_a = p->h_tiles;
+    p->ntiles_a = p->ntiles;
+    p->RHmax_a = p->RHmax;
+    p->RWmax_a = p->RWmax;
+    p->tb_a = atb;
+    p->nt_a = ant;
+    p->cpt_a = acp;
+    tun.a_strips = as; tun.a_tcols = ac; tun.a_tblock = atb; tun.a_nt = ant; tun.a_cpt = acp;
+    p->h_tiles = ft;
+    p->ntiles = fnt;
+    p->RHmax = fRH;
+    p->RWmax = fRW;
+    p->LWmax = fLW;
+  } else {
+    p->h_tiles_a = p->h_tiles;
+    p->ntiles_a = p->ntiles;
+    p->RHmax_a = p->RHmax;
+    p->RWmax_a = p->RWmax;
+    p->tb_a = tun.tblock;
+    p->nt_a = tun.nt;
+    p->cpt_a = tun.cpt;
+    tun.a_strips = tun.strips; tun.a_tcols = tun.tcols; tun.a_tblock = tun.tblock;
+    tun.a_nt = tun.nt; tun.a_cpt = tun.cpt;
+  }
   p->tun = tun;
   p->tiled = tiled;
   plan_layout(*p);
@@ -587,7 +640,7 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
   int rc;
   if ((rc = up(&p->d_spx, p->h_spx)) || (rc = up(&p->d_taps, p->h_taps)) ||
       (rc = up(&p->d_spmap, spmap)) || (rc = up(&p->d_mult, p->h_mult)) ||
-      (rc = up(&p->d_tiles, p->h_tiles))) {
+      (rc = up(&p->d_tiles, p->h_tiles)) || (rc = up(&p->d_tiles_a, p->h_tiles_a))) {
     free_plan(p);
     return rc;
   }
@@ -690,16 +743,23 @@ FwdArgs base_fwd(const Plan &p, const Ws &w, const float *b, const float *c, con
   return a;
 }
 
-// Runs forward steps [n0, n1) starting from state (s1 = u_n0, s2 = u_{n0-1}).
-// Tiled: ping-pong between (s1,s2) and (o1,o2); plain: rotation through
-// three buffers.  Returns the buffers holding the final state.
-int run_forward(const Plan &p, FwdArgs a, int n0, int n1, float *bufs[6], float **fin1,
-                float **fin2, bool rec, hipStream_t s) {
+// Runs forward steps [n0, n1) from the state (in1, in2) = (u_n0, u_{n0-1}).
+// Tiled: launches of tblock steps; the first reads (in1, in2), intermediate
+// states ping-pong through tmp[0..3], and the last launch writes (out1, out2)
+// when given.  Plain: one launch per step rotating through tmp[0..2]; the
+// final state is copied to (out1, out2) when given.
+int run_forward(const Plan &p, FwdArgs a, int n0, int n1, const float *in1, const float *in2,
+                float *out1, float *out2, float *tmp[4], bool rec, hipStream_t s) {
+  const int64_t BF = (int64_t)p.B * p.F;
   if (p.tiled) {
-    float *i1 = bufs[0], *i2 = bufs[1], *o1 = bufs[2], *o2 = bufs[3];
+    const float *i1 = in1, *i2 = in2;
+    int pp = 0;
     int n = n0;
     while (n < n1) {
       int k = std::min(p.tun.tblock, n1 - n);
+      bool last = n + k >= n1;
+      float *o1 = (last && out1) ? out1 : tmp[2 * pp];
+      float *o2 = (last && out2) ? out2 : tmp[2 * pp + 1];
       a.in1 = i1; a.in2 = i2; a.out1 = o1; a.out2 = o2;
       a.n0 = n; a.nsteps = k;
       ProfScope ps(rec ? 1 : 0, s, (int64_t)p.B * p.Nx * p.Ny * k);
@@ -707,23 +767,25 @@ int run_forward(const Plan &p, FwdArgs a, int n0, int n1, float *bufs[6], float 
         WT_HIP(launch_fwd_tiled(p.tun.nt, p.tun.cpt, p.unit_h, p.nl, rec, a, p.LWmax, s));
       else
         WT_HIP(launch_fwd_cs(p.tun.nt, p.tun.cpt, p.nl, rec, a, p.RHmax, p.RWmax, s));
-      std::swap(i1, o1);
-      std::swap(i2, o2);
+      i1 = o1;
+      i2 = o2;
+      pp ^= 1;
       n += k;
     }
-    *fin1 = i1;
-    *fin2 = i2;
   } else {
-    // u_n in r1, u_{n-1} in r2, output into r3
-    float *r1 = bufs[0], *r2 = bufs[1], *r3 = bufs[2];
+    // u_n in r1, u_{n-1} in r2, output into r3; the inputs are copied into
+    // the rotation buffers first so the caller's state is never written
+    float *r1 = tmp[0], *r2 = tmp[1], *r3 = tmp[2];
+    WT_HIP(hipMemcpyAsync(r1, in1, BF * sizeof(float), hipMemcpyDeviceToDevice, s));
+    WT_HIP(hipMemcpyAsync(r2, in2, BF * sizeof(float), hipMemcpyDeviceToDevice, s));
     for (int n = n0; n < n1; ++n) {
       a.in1 = r1; a.in2 = r2; a.out1 = r3;
       ProfScope ps(rec ? 1 : 0, s, (int64_t)p.B * p.Nx * p.Ny);
       WT_HIP(launch_fwd_plain(p.unit_h, p.nl, rec, a, p.d_spmap, n, s));
       float *t = r2; r2 = r1; r1 = r3; r3 = t;
     }
-    *fin1 = r1;
-    *fin2 = r2;
+    if (out1) WT_HIP(hipMemcpyAsync(out1, r1, BF * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (out2) WT_HIP(hipMemcpyAsync(out2, r2, BF * sizeof(float), hipMemcpyDeviceToDevice, s));
   }
   return WT_OK;
 }
@@ -758,35 +820,34 @@ int wt_loop_fwd(void *plan, const float *b, const float *c, const float *rho, co
   hipStream_t s = (hipStream_t)stream;
   Ws w = layout(*p, saved, temp, with_grad != 0);
   const int64_t BF = (int64_t)p->B * p->F;
-  float *bufs[6];
-  for (int i = 0; i < 6; ++i) bufs[i] = w.st + i * BF;
-  // clamp-only: clamped c goes into the last state buffer (unused by the loop)
-  int rc = prepare_coef(*p, w, b, c, rho, bufs[5], s);
+  float *bufs[8];
+  for (int i = 0; i < 8; ++i) bufs[i] = w.st + i * BF;
+  // clamp-only: the clamped c goes into a state buffer before the loop uses it
+  int rc = prepare_coef(*p, w, b, c, rho, bufs[7], s);
   if (rc) return rc;
   FwdArgs a = base_fwd(*p, w, b, c, rho, x);
   a.y = y;
   a.yraw = with_grad ? w.yraw : nullptr;
   a.fields = fields;
   a.fstride = field_stride;
-  WT_HIP(launch_fill(bufs[0], 2 * BF, 0.0f, s));  // u_0 = u_{-1} = 0 (rnn.py:39-41)
-  float *cur1 = bufs[0], *cur2 = bufs[1];
+  // state pairs: bufs[0..3] intermediate ping-pong, bufs[4..7] segment
+  // input/output when no checkpoints are kept.  u_0 = u_{-1} = 0
+  // (rnn.py:39-41) lives in checkpoint 0 when recording.
+  float *tmp[4] = {bufs[0], bufs[1], bufs[2], bufs[3]};
+  float *z = with_grad ? w.ck : bufs[4];
+  WT_HIP(launch_fill(z, 2 * BF, 0.0f, s));
+  const float *cur1 = z, *cur2 = z + BF;
   for (int sg = 0; sg < p->nseg; ++sg) {
     int n0 = sg * p->K, n1 = std::min(p->T, n0 + p->K);
-    if (with_grad) {
-      float *ck = w.ck + (size_t)sg * 2 * BF;
-      WT_HIP(hipMemcpyAsync(ck, cur1, BF * sizeof(float), hipMemcpyDeviceToDevice, s));
-      WT_HIP(hipMemcpyAsync(ck + BF, cur2, BF * sizeof(float), hipMemcpyDeviceToDevice, s));
-    }
-    // keep the live state in bufs[0..1] so the ping-pong partners are fixed
-    float *set[6] = {cur1, cur2, nullptr, nullptr, nullptr, nullptr};
-    int nfree = 2;
-    for (int i = 0; i < 5 && nfree < 6; ++i)
-      if (bufs[i] != cur1 && bufs[i] != cur2) set[nfree++] = bufs[i];
-    float *f1, *f2;
-    rc = run_forward(*p, a, n0, n1, set, &f1, &f2, false, s);
+    float *o1;
+    if (with_grad && sg + 1 < p->nseg)
+      o1 = w.ck + (size_t)(sg + 1) * 2 * BF;
+    else
+      o1 = (cur1 == bufs[4]) ? bufs[6] : bufs[4];
+    rc = run_forward(*p, a, n0, n1, cur1, cur2, o1, o1 + BF, tmp, false, s);
     if (rc) return rc;
-    cur1 = f1;
-    cur2 = f2;
+    cur1 = o1;
+    cur2 = o1 + BF;
   }
   return WT_OK;
 }
@@ -808,8 +869,8 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
   const int64_t BF = (int64_t)p->B * p->F;
   const int T = p->T, K = p->K;
   bool use_tiled = p->tiled && fbar == nullptr;
-  float *st[6];
-  for (int i = 0; i < 6; ++i) st[i] = w.st + i * BF;
+  float *st[8];
+  for (int i = 0; i < 8; ++i) st[i] = w.st + i * BF;
   float *lam[4];
   for (int i = 0; i < 4; ++i) lam[i] = w.lam + i * BF;
   WT_HIP(launch_fill(w.lam, 4 * BF, 0.0f, s));  // lambda_{T+1} = lambda_{T+2} = 0
@@ -822,7 +883,7 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
   fa.rec_K = K;
   AdjArgs aa{};
   aa.B = p->B; aa.Nx = p->Nx; aa.Ny = p->Ny; aa.T = T;
-  aa.ntiles = p->ntiles; aa.tiles = p->d_tiles; aa.spx = p->d_spx; aa.taps = p->d_taps;
+  aa.ntiles = p->ntiles_a; aa.tiles = p->d_tiles_a; aa.spx = p->d_spx; aa.taps = p->d_taps;
   aa.ntap = p->ntap; aa.nsrcslot = std::max(p->nslot, 1);
   aa.A = w.A; aa.Bm = w.Bm; aa.C2 = w.C2;
   aa.s2 = p->s2; aa.dtm1 = p->dtm1; aa.dtm2 = p->dtm2; aa.t1 = p->t1; aa.t4 = p->t4;
@@ -836,17 +897,16 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
   for (int sg = p->nseg - 1; sg >= 0; --sg) {
     int n0 = sg * K, n1 = std::min(T, n0 + K);
     // recompute the segment from its checkpoint, recording lap(u_n)
-    float *ck = w.ck + (size_t)sg * 2 * BF;
-    WT_HIP(hipMemcpyAsync(st[0], ck, 2 * BF * sizeof(float), hipMemcpyDeviceToDevice, s));
+    const float *ck = w.ck + (size_t)sg * 2 * BF;
     fa.rec_n0 = n0;
-    float *f1, *f2;
+    float *tmp[4] = {st[0], st[1], st[2], st[3]};
     int rc;
     if (use_tiled) {
-      rc = run_forward(*p, fa, n0, n1, st, &f1, &f2, true, s);
+      rc = run_forward(*p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
     } else {
       Plan q = *p;
       q.tiled = false;
-      rc = run_forward(q, fa, n0, n1, st, &f1, &f2, true, s);
+      rc = run_forward(q, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
     }
     if (rc) return rc;
     aa.seg_n0 = n0;
@@ -854,14 +914,14 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
     int m = n1;
     if (use_tiled) {
       while (m > n0) {
-        int k = std::min(p->tun.tblock, m - n0);
+        int k = std::min(p->tb_a, m - n0);
         aa.in1 = l1; aa.in2 = l2; aa.out1 = l3; aa.out2 = l4;
         aa.m_hi = m; aa.nsteps = k;
         ProfScope ps(2, s, (int64_t)p->B * p->Nx * p->Ny * k);
         if (p->tun.kind == 1)
           WT_HIP(launch_adj_tiled(p->tun.nt, p->tun.cpt, p->unit_h, aa, p->LWmax, s));
         else
-          WT_HIP(launch_adj_cs(p->tun.nt, p->tun.cpt, aa, p->RHmax, p->RWmax, s));
+          WT_HIP(launch_adj_cs(p->nt_a, p->cpt_a, aa, p->RHmax_a, p->RWmax_a, s));
         std::swap(l1, l3);
         std::swap(l2, l4);
         m -= k;
