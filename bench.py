@@ -33,8 +33,18 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12
 BYTES = {0: 12.0, 1: 12.0, 2: 16.0}  # algorithmic bytes per cell-step per kernel class
-CLASS_NAME = {0: "k_fwd_tiled (forward)", 1: "k_fwd_tiled (recompute, record lap)",
-              2: "k_adj_tiled (adjoint)"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_train_150_b64.json")
+
+
+def class_name(cls, tun):
+    """Kernel (as rocprofv3 names it) of a kernel class under plan tuning `tun`."""
+    if tun["plain"]:
+        return ["wt::k_fwd_plain", "wt::k_fwd_plain (record)", "wt::k_adj_plain"][cls]
+    if tun["kind"] == 1:
+        return ["wt::k_fwd_tiled", "wt::k_fwd_tiled (record)", "wt::k_adj_tiled"][cls]
+    if cls == 2:
+        return f"wt::k_adj_cs<{tun['a_nt']}, {tun['a_cpt']}>"
+    return f"wt::k_fwd_cs<{tun['nt']}, {tun['cpt']}, 0, {'true' if cls == 1 else 'false'}>"
 
 
 def log(*a):
@@ -88,6 +98,65 @@ def time_steps(model, x, labels, steps, warmup, profile=False):
     return dt, float(loss.item()), prof
 
 
+def traffic(kname, tun):
+    """Measured HBM bytes per launch of `kname` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary written by --summarize-pmc, when it was taken on the
+    same kernel and tuning; else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = d.get("kernels", {}).get(kname)
+    if k is None or d.get("tuning") != tun:
+        return None
+    return k["hbm_bytes_per_launch"]
+
+
+def summarize_pmc(fetch_dir, write_dir, calib_bytes):
+    """Per-kernel average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
+    rocprofv3 --pmc runs, each scaled by what the counter reads for the
+    bench's own device copy of `calib_bytes` bytes; writes TRAFFIC_FILE."""
+    import collections
+    import csv
+    import glob
+
+    def read(d, counter):
+        agg = collections.defaultdict(list)
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] == counter:
+                    agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+        return {k: sum(v) / len(v) for k, v in agg.items()}
+
+    fe, wr = read(fetch_dir, "FETCH_SIZE"), read(write_dir, "WRITE_SIZE")
+    copy = [k for k in fe if "copy" in k.lower() or "elementwise" in k.lower()]
+    if not copy:
+        raise SystemExit("no copy kernel in the FETCH_SIZE run (run bench.py with --calib-copy)")
+    ck = max(copy, key=lambda k: fe[k])
+    f_scale = calib_bytes / (fe[ck] * 1024.0)
+    w_scale = calib_bytes / (wr[ck] * 1024.0)
+    meta = json.load(open(os.path.join(fetch_dir, "bench_meta.json")))
+    out = {"tuning": meta["tuning"], "calibration": {
+        "kernel": ck, "bytes": calib_bytes, "fetch_kib": fe[ck], "write_kib": wr[ck],
+        "fetch_scale": f_scale, "write_scale": w_scale}, "kernels": {}}
+    lines = [f"copy calibration: {ck}: {calib_bytes} B -> FETCH {fe[ck]:.1f} KiB "
+             f"(scale {f_scale:.4f}), WRITE {wr[ck]:.1f} KiB (scale {w_scale:.4f})"]
+    for k in sorted(fe, key=lambda k: -fe[k]):
+        if k == ck or k not in wr:
+            continue
+        name = k.split("(")[0].strip()
+        name = name[5:] if name.startswith("void ") else name
+        fb, wb = fe[k] * 1024.0 * f_scale, wr[k] * 1024.0 * w_scale
+        out["kernels"][name] = {"fetch_bytes": fb, "write_bytes": wb,
+                                "hbm_bytes_per_launch": fb + wb}
+        lines.append(f"{name}: fetch {fb / 1e6:.3f} MB  write {wb / 1e6:.3f} MB per launch")
+    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(out, f, indent=1)
+    print("\n".join(lines))
+
+
 def cpu_baseline(budget_s=15.0):
     """The reference's op sequence (torch-CPU restatement, fp32) on the same
     workload with T cut to fit ~budget_s of forward+backward."""
@@ -134,7 +203,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-kernel comparison")
     ap.add_argument("--tuning", type=str, default=None, help="JSON wt_tuning override")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="after the timed region, copy MIB MiB device-to-device 3 times (the "
+                         "known-size copy that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--meta-out", type=str, default=None,
+                    help="write the resolved tuning to this JSON file")
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "BYTES"),
+                    help="summarise two rocprofv3 --pmc runs into the traffic file and exit")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], int(args.summarize_pmc[2]))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(asked for --gpus {args.gpus}); run with --gpus 1", file=sys.stderr)
@@ -161,8 +240,9 @@ def main():
     per_launch = work / max(nl, 1)
     achieved = BYTES[cls] * per_launch / avg_s
     for c, (m_, n_, w_) in prof.items():
-        log(f"[bench] class {c} {CLASS_NAME[c]}: {n_} launches, {m_:.2f} ms total, "
+        log(f"[bench] class {c} {class_name(c, plan.tuning)}: {n_} launches, {m_:.2f} ms total, "
             f"{(m_ / max(n_, 1)) * 1e3:.1f} us avg")
+    kname = class_name(cls, plan.tuning)
     out = {
         "metric": "fwd+adjoint Gcell-steps/s (150x150, batch 64, 3000 steps, training step)",
         "value": round(value, 3),
@@ -180,13 +260,27 @@ def main():
                                "on the 110x110 interior, batch 64, 3000 steps, 3 intensity "
                                "probes, CE loss, adjoint with checkpoints every 64 steps, Adam",
                    "B": B, "Nx": N, "Ny": N, "T": T, "tuning": plan.tuning},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None,
-                     "kernel": CLASS_NAME[cls], "bytes_per_cell_step": BYTES[cls],
-                     "cell_steps_per_launch": per_launch, "avg_launch_us": avg_s * 1e6,
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
+                     "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
+                     "traffic": traffic(kname, plan.tuning),
+                     "kernel": kname, "bytes_per_cell_step": BYTES[cls],
+                     "cell_steps_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
                      "launches": nl},
-        "whole_step_alg_GBps": 28.0 * cells / dt / 1e9,
+        # the whole training step at 28 B per cell-step (12 forward + 16 adjoint)
+        "step_roofline": {"achieved": round(28.0 * cells / dt / 1e9, 1), "unit": "GB/s",
+                          "frac": round(28.0 * cells / dt / PEAK_HBM, 4)},
     }
+    if args.meta_out:
+        with open(args.meta_out, "w") as f:
+            json.dump({"tuning": plan.tuning}, f)
+    if args.calib_copy > 0:
+        n = args.calib_copy * 2 ** 20 // 4
+        a = torch.ones(n, device=dev)
+        bb = torch.empty_like(a)
+        for _ in range(3):
+            bb.copy_(a)
+        torch.cuda.synchronize()
+        del a, bb
     if not args.no_plain:
         pm, px, pl = build(dev, {"plain": 1})
         pdt, _, _ = time_steps(pm, px, pl, 1, 1)

@@ -133,6 +133,14 @@ typedef struct wt_tuning {
  * wt_plan_destroy.  `tuning` may be NULL. */
 int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax,
                    const wt_tuning *tuning, void **plan_out);
+
+/* Host-only half of wt_plan_create: the same validation (including the CFL
+ * check of cell.py:66-73) and tiling/variant choice, with no device work.
+ * Writes the resolved tuning to *out and the with-grad workspace sizes of
+ * wt_workspace_bytes to *saved / *temp (any may be NULL). */
+int wt_plan_resolve(const wt_config *cfg, int32_t B, int32_t T, float cmax,
+                    const wt_tuning *tuning, wt_tuning *out, int64_t *saved,
+                    int64_t *temp);
 int wt_plan_destroy(void *plan);
 /* The tuning actually chosen (after heuristics). */
 int wt_plan_tuning(const void *plan, wt_tuning *out);

@@ -1,0 +1,216 @@
+"""CPU tests of the host side: the C ABI library loads and exports every
+declared symbol, host-only plan resolution (validation, CFL, tiling
+heuristics, workspace sizes), and the reference's module API surface
+(constructors, buffers, state_dict keys, errors).  No kernel is launched."""
+from __future__ import annotations
+
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from wavetorch_amd import _lib
+from wavetorch_amd._loop import LoopConfig, resolve
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = []
+    for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names += re.findall(r"^int\s+(wt_\w+)\s*\(", txt, flags=re.M)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in include/ but not exported"
+    # every binding in the ctypes table is declared in the header
+    assert set(_lib._SIGS) <= set(names)
+    assert L.wt_abi_version() == _lib.ABI_VERSION
+
+
+def test_last_error_is_nul_terminated():
+    cfg = LoopConfig(Nx=2, Ny=2, dt=0.5, h=1.0, src_px=(), taps=())
+    with pytest.raises(ValueError, match="invalid shape"):
+        resolve(cfg, 1, 10, 1.0)
+    import ctypes
+    buf = ctypes.create_string_buffer(8)
+    n = _lib.lib().wt_last_error(buf, 8)
+    assert n > 8 and len(buf.value) == 7
+
+
+def c2_cfg(N=150):
+    return LoopConfig(Nx=N, Ny=N, dt=0.707, h=1.0, src_px=((40, 75, 0),),
+                      taps=((110, 55, 1), (110, 75, 1), (110, 95, 1)))
+
+
+def test_resolve_training_shape_heuristic():
+    """Batch 64 on 150x150: 2x2 tiles per sample (256 workgroups), 16-step
+    forward blocks at 10 rows per thread, 12-step adjoint blocks at 8."""
+    t, saved, temp = resolve(c2_cfg(), 64, 3000, 1.0)
+    assert (t["plain"], t["kind"], t["strips"], t["tcols"]) == (0, 0, 2, 2)
+    assert (t["tblock"], t["nt"], t["cpt"]) == (16, 1024, 10)
+    assert (t["a_strips"], t["a_tcols"], t["a_tblock"], t["a_nt"], t["a_cpt"]) == \
+        (2, 2, 12, 1024, 8)
+    assert t["ck_every"] == 64
+    F = 150 * 150
+    # checkpoints (u_n, u_{n-1}) for ceil(T/K) segments dominate the saved bytes
+    assert saved >= math.ceil(3000 / 64) * 2 * 64 * F * 4
+    assert saved < math.ceil(3000 / 64) * 2 * 64 * F * 4 + 8 * 2 ** 20
+
+
+def test_resolve_keeps_explicit_tuning():
+    tun = {"strips": 6, "tcols": 1, "tblock": 8, "nt": 1024, "cpt": 8, "a_tblock": 8}
+    t, _, _ = resolve(c2_cfg(), 8, 200, 1.0, tun)
+    assert (t["strips"], t["tcols"], t["tblock"], t["nt"], t["cpt"]) == (6, 1, 8, 1024, 8)
+    assert t["a_tblock"] == 8 and (t["a_nt"], t["a_cpt"]) == (1024, 8)
+    t, _, _ = resolve(c2_cfg(), 8, 200, 1.0, {"plain": 1})
+    assert t["plain"] == 1 and t["tblock"] == 1
+
+
+def test_resolve_rejects_infeasible_tuning():
+    # 1 strip of 150 rows at 4 rows per thread needs 38 x 150 threads
+    with pytest.raises(RuntimeError, match="does not fit"):
+        resolve(c2_cfg(), 8, 200, 1.0, {"strips": 1, "tcols": 1, "nt": 1024, "cpt": 4})
+
+
+def test_resolve_cfl_and_indices():
+    with pytest.raises(ValueError, match="CFL"):
+        resolve(c2_cfg(), 4, 100, 1.5)  # dt 0.707 > 1/(1.5*sqrt 2)
+    bad = LoopConfig(Nx=50, Ny=50, dt=0.5, h=1.0, src_px=((50, 3, 0),), taps=())
+    with pytest.raises(ValueError):
+        resolve(bad, 1, 10, 1.0)
+    bad = LoopConfig(Nx=50, Ny=50, dt=0.5, h=1.0, src_px=((3, 3, 0),), taps=((3, -1, 1),))
+    with pytest.raises(ValueError):
+        resolve(bad, 1, 10, 1.0)
+
+
+def test_checkpoint_memory_vs_K():
+    """Saved + temporary bytes of the nonlinear 150x150 batch-32 loop for K in
+    {16, 64, 256}: checkpoints scale as T/K, the segment buffer as K."""
+    cfg = LoopConfig(Nx=150, Ny=150, dt=0.64, h=1.0, src_px=((40, 75, 0),),
+                     taps=((110, 55, 1), (110, 75, 1), (110, 95, 1)),
+                     sat_b0=0.5, sat_uth=math.sqrt(0.1), c_nl=0.05, c_clamp=(0.4, 1.1))
+    F = 32 * 150 * 150 * 4
+    tot = {}
+    for K in (16, 64, 256):
+        _, saved, temp = resolve(cfg, 32, 3000, 1.0, {"ck_every": K})
+        assert saved >= math.ceil(3000 / K) * 2 * F
+        tot[K] = saved + temp
+    assert tot[64] < tot[16] and tot[64] < tot[256]
+    assert tot[64] < 1.2e9  # vs ~8.6 GB for every field of the 3000 steps
+
+
+# ---------------------------------------------------------------- module API
+
+def _model():
+    from wavetorch_amd import (WaveCell, WaveGeometryFreeForm, WaveIntensityProbe, WaveRNN,
+                               WaveSource)
+    geom = WaveGeometryFreeForm((60, 50), 1.0, c0=1.0, c1=0.5, abs_N=10, abs_sig=1.0,
+                                abs_p=2.0, rho="rand")
+    cell = WaveCell(0.5, geom)
+    return WaveRNN(cell, [WaveSource(20, 25)],
+                   [WaveIntensityProbe(40, 10), WaveIntensityProbe(40, 30)])
+
+
+def test_state_dict_keys_match_reference_order():
+    keys = list(_model().state_dict().keys())
+    want = ["cell.dt", "cell.satdamp_b0", "cell.satdamp_uth", "cell.c_nl",
+            "cell.geom.rho", "cell.geom.h", "cell.geom.c0", "cell.geom.c1", "cell.geom.abs_N",
+            "cell.geom.abs_sig", "cell.geom.abs_p", "cell.geom._b", "cell.geom.eta",
+            "cell.geom.beta", "cell.geom.design_region", "cell.geom.blur_kernel",
+            "cell.geom.blur_N", "cell.geom.blur_radius",
+            "sources.0.x", "sources.0.y", "probes.0.x", "probes.0.y", "probes.1.x",
+            "probes.1.y"]
+    assert sorted(keys) == sorted(want)
+    sd = _model().state_dict()
+    assert sd["cell.geom.abs_N"].dtype == torch.uint8
+    assert sd["cell.geom.abs_p"].dtype == torch.uint8
+    assert sd["cell.geom.design_region"].dtype == torch.uint8
+    assert sd["cell.geom.blur_N"].dtype == torch.int32
+    assert sd["sources.0.x"].dtype == torch.int64
+    assert tuple(sd["cell.geom.blur_kernel"].shape) == (1, 1, 3, 3)
+    # blur radius 1 selects the full 3x3 (a 1/9 box)
+    assert torch.allclose(sd["cell.geom.blur_kernel"], torch.full((1, 1, 3, 3), 1 / 9))
+
+
+def test_cell_cfl_raises_value_error():
+    from wavetorch_amd import WaveCell, WaveGeometryFreeForm
+    geom = WaveGeometryFreeForm((40, 40), 1.0, c0=1.0, c1=0.5, abs_N=5)
+    WaveCell(0.70, geom)
+    with pytest.raises(ValueError, match="CFL"):
+        WaveCell(0.71, geom)
+
+
+def test_parameters_are_geometry_parameters():
+    m = _model()
+    ps = list(m.parameters())
+    assert len(ps) == 1 and ps[0] is m.cell.geom.rho
+
+
+def test_rho_constrained_to_design_region_and_pml():
+    m = _model()
+    g = m.cell.geom
+    assert torch.all(g.rho.detach()[g.b > 0] == 0)
+
+
+def test_product_path_has_no_cpu_fallback():
+    m = _model()
+    with pytest.raises(RuntimeError, match="HIP device only"):
+        m(torch.zeros(1, 10))
+    from wavetorch_amd import TimeStep
+    z = torch.zeros(1, 8, 8)
+    with pytest.raises(RuntimeError, match="HIP device only"):
+        TimeStep.apply(torch.zeros(8, 8), torch.ones(8, 8), z, z, torch.tensor(0.5),
+                       torch.tensor(1.0))
+
+
+def test_line_source_axis_aligned_and_diagonal():
+    from wavetorch_amd import WaveLineSource
+    s = WaveLineSource(5, 3, 5, 9)
+    assert s.pixels() == [(5, c) for c in range(3, 10)]
+    s = WaveLineSource(10, 4, 2, 4)
+    assert s.pixels() == [(r, 4) for r in range(10, 1, -1)]
+    s = WaveLineSource(0, 0, 4, 4)
+    assert s.pixels() == [(i, i) for i in range(5)]
+    # a shallow line: one pixel per column, endpoints included, rows monotone
+    s = WaveLineSource(0, 0, 3, 10)
+    px = s.pixels()
+    assert len(px) == 11 and px[0] == (0, 0) and px[-1] == (3, 10)
+    rows = [p[0] for p in px]
+    assert rows == sorted(rows) and all(b - a <= 1 for a, b in zip(rows, rows[1:]))
+
+
+def test_probe_taps_order_matches_stack():
+    """torch.stack(dim=-1) of multi-pixel probes: column = l * P + p."""
+    from wavetorch_amd import (WaveCell, WaveGeometryFreeForm, WaveProbe, WaveIntensityProbe,
+                               WaveRNN, WaveSource)
+    geom = WaveGeometryFreeForm((40, 40), 1.0, c0=1.0, c1=0.5, abs_N=5)
+    m = WaveRNN(WaveCell(0.5, geom), WaveSource(10, 10),
+                [WaveProbe([30, 31], [5, 6]), WaveIntensityProbe([30, 31], [20, 21])])
+    cfg = m.loop_config()
+    assert cfg.taps == ((30, 5, 0), (30, 20, 1), (31, 6, 0), (31, 21, 1))
+    assert cfg.src_px == ((10, 10, 0),)
+
+
+def test_pml_profile_closed_form():
+    from wavetorch_amd import WaveGeometryFreeForm
+    N, aN, sig, p = 50, 8, 1.3, 2
+    g = WaveGeometryFreeForm((N, N), 1.0, c0=1.0, c1=0.5, abs_N=aN, abs_sig=sig, abs_p=p)
+    b = g.b.numpy().astype(np.float64)
+    ramp = sig * (np.linspace(0, 1, aN + 1) ** p)[::-1]
+    bx = np.zeros(N)
+    bx[:aN + 1] = ramp
+    bx[N - aN - 1:] = ramp[::-1]
+    want = np.sqrt(bx[:, None] ** 2 + bx[None, :] ** 2)
+    np.testing.assert_allclose(b, want, rtol=1e-6, atol=1e-7)
+    assert b[aN, aN] == 0 and abs(b[0, 0] - math.sqrt(2) * sig) < 1e-6

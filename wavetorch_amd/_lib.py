@@ -64,6 +64,9 @@ _SIGS = {
                                    c_f32, c_p, c_p, c_p, c_p, c_p]),
     "wt_plan_create": (ctypes.c_int, [ctypes.POINTER(WtConfig), c_i32, c_i32, c_f32,
                                       ctypes.POINTER(WtTuning), ctypes.POINTER(c_p)]),
+    "wt_plan_resolve": (ctypes.c_int, [ctypes.POINTER(WtConfig), c_i32, c_i32, c_f32,
+                                       ctypes.POINTER(WtTuning), ctypes.POINTER(WtTuning),
+                                       ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
     "wt_plan_destroy": (ctypes.c_int, [c_p]),
     "wt_plan_tuning": (ctypes.c_int, [c_p, ctypes.POINTER(WtTuning)]),
     "wt_workspace_bytes": (ctypes.c_int, [c_p, c_i32, ctypes.POINTER(c_i64),

@@ -51,6 +51,23 @@ class LoopConfig:
         return cfg, (src, taps)  # keep the arrays alive while cfg is used
 
 
+def resolve(cfg: LoopConfig, B: int, T: int, cmax: float, tuning=None):
+    """Host-only plan resolution (``wt_plan_resolve``): validates ``cfg``
+    (ValueError on CFL or bad indices, as the reference's constructors raise)
+    and returns (resolved tuning dict, saved bytes, temp bytes) without
+    touching a device."""
+    c, keep = cfg.to_c()
+    tun = _lib.WtTuning(**{k: int(v) for k, v in tuning.items()}) if tuning else None
+    out = _lib.WtTuning()
+    s0, t0 = _lib.c_i64(), _lib.c_i64()
+    _lib.check(_lib.lib().wt_plan_resolve(ctypes.byref(c), B, T, float(cmax),
+                                          ctypes.byref(tun) if tun is not None else None,
+                                          ctypes.byref(out), ctypes.byref(s0), ctypes.byref(t0)),
+               "wt_plan_resolve")
+    del keep
+    return out.as_dict(), s0.value, t0.value
+
+
 class Plan:
     """Owner of a library plan (validated config + device tables + tiling)."""
 

@@ -401,8 +401,12 @@ int wt_step_bwd(const float *b, const float *c, int32_t bc_per_sample, const flo
   return WT_OK;
 }
 
-int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
-                   const wt_tuning *tuning, void **plan_out) {
+}  // extern "C"
+
+// Host half of plan creation: validation, special-pixel tables, tiling and
+// variant choice, workspace layout.  No device call except hipGetDevice.
+static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
+                      const wt_tuning *tuning, Plan **plan_out) {
   if (!cfg || !plan_out) {
     set_error("wt_plan_create: null argument");
     return WT_EINVAL;
@@ -528,18 +532,25 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
     if (tb <= 0) tb = 16;
     if (tb > p->K) tb = p->K;
     if (tun.kind == 0 && tb > 256) tb = 256;  // inputs/seeds of a launch are staged in LDS
-    if (strips <= 0) {
-      // heuristic: enough workgroups to cover the 256 CUs, region fitting
-      // the largest variant
+    // Heuristic for the column-segment family: g x g square tiles per
+    // sample, g the smallest giving one workgroup per CU (B*g*g >= 256)
+    // while a tile's own rows stay at least twice the halo; grown further
+    // only if no block variant holds the tile.
+    bool square = tun.kind == 0 && strips <= 0 && tun.tcols <= 0;
+    if (square) {
+      strips = 1;
+      while ((int64_t)B * strips * strips < 256 &&
+             std::min(p->Nx, p->Ny) / (strips + 1) >= 2 * tb)
+        ++strips;
+    } else if (strips <= 0) {
+      // row strips: enough workgroups to cover the 256 CUs
       strips = 1;
       while (strips < p->Nx / 4 && (int64_t)B * strips < 256) ++strips;
     }
     int tcols = (tun.kind == 0 && tun.tcols > 0) ? tun.tcols : 1;
     bool ok = false;
     for (; strips <= p->Nx; ++strips) {
-      if (strips > 1 && tb > (p->Nx / strips)) {
-        // a halo deeper than the strip is allowed but wasteful; keep going
-      }
+      if (square) tcols = std::min(strips, p->Ny);
       build_tiles(*p, strips, tcols, tb);
       if (nt > 0 && cpt > 0) {
         ok = tun.kind == 1 ? tiles_ok(*p, nt, cpt) : tiles_ok_cs(*p, nt, cpt);
@@ -581,13 +592,20 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
     int fnt = p->ntiles, fRH = p->RHmax, fRW = p->RWmax, fLW = p->LWmax;
     int as = tun.a_strips > 0 ? tun.a_strips : tun.strips;
     int ac = tun.a_tcols > 0 ? tun.a_tcols : (tun.tcols > 0 ? tun.tcols : 1);
-    int atb = tun.a_tblock > 0 ? tun.a_tblock : tun.tblock;
+    // the adjoint holds three fields per row in registers: a shallower
+    // default block keeps its tile within 8 rows per thread
+    int atb = tun.a_tblock > 0 ? tun.a_tblock
+                               : (tuning && tuning->tblock > 0 ? tun.tblock
+                                                               : std::min(tun.tblock, 12));
     if (atb > p->K) atb = p->K;
     if (atb > 256) atb = 256;
     int ant = tun.a_nt > 0 ? tun.a_nt : tun.nt;
     int acp = tun.a_cpt > 0 ? tun.a_cpt : tun.cpt;
     build_tiles(*p, as, ac, atb);
-    bool aok = tiles_ok_cs(*p, ant, acp);
+    // an explicit variant (adjoint's own, or the forward's) is tried as is;
+    // otherwise the fewest rows per thread that fit
+    bool aok = false;
+    if (tun.a_nt > 0 || (tuning && tuning->nt > 0)) aok = tiles_ok_cs(*p, ant, acp);
     if (!aok && tun.a_nt <= 0) aok = pick_variant_cs(*p, ant, acp);
     if (!aok) {
       set_error("wt_plan_create: adjoint tuning (strips=%d tcols=%d tblock=%d nt=%d cpt=%d) "
@@ -622,6 +640,33 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
   p->tun = tun;
   p->tiled = tiled;
   plan_layout(*p);
+  *plan_out = p;
+  return WT_OK;
+}
+
+extern "C" {
+
+int wt_plan_resolve(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
+                    const wt_tuning *tuning, wt_tuning *out, int64_t *saved, int64_t *temp) {
+  Plan *p = nullptr;
+  int rc = plan_build(cfg, B, T, cmax_geom, tuning, &p);
+  if (rc) return rc;
+  if (out) *out = p->tun;
+  if (saved) *saved = (int64_t)p->saved_bytes;
+  if (temp) *temp = (int64_t)p->temp_bytes;
+  free_plan(p);
+  return WT_OK;
+}
+
+int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
+                   const wt_tuning *tuning, void **plan_out) {
+  if (!plan_out) {
+    set_error("wt_plan_create: null argument");
+    return WT_EINVAL;
+  }
+  Plan *p = nullptr;
+  int rc0 = plan_build(cfg, B, T, cmax_geom, tuning, &p);
+  if (rc0) return rc0;
 
   // ---- device tables
   auto up = [&](int32_t **dst, const std::vector<int32_t> &v) -> int {
