@@ -72,15 +72,17 @@ def train_step(model, x, labels, opt):
 
 
 def time_steps(model, x, labels, steps, warmup, profile=False):
+    """Times `steps` training steps after `warmup`.  With `profile`, a second
+    timed region of the same length runs with the library's HIP-event timing
+    of every loop-kernel launch (events recorded on the launch stream); its
+    per-class sums give the roofline, while the first region (no events)
+    gives the step time."""
     from wavetorch_amd import _lib
     opt = torch.optim.Adam(model.parameters(), lr=1e-3)
     for _ in range(warmup):
         train_step(model, x, labels, opt)
     torch.cuda.synchronize()
     L = _lib.lib()
-    if profile:
-        L.wt_profile(1)
-        _lib.check(L.wt_profile_read(None, None, None), "wt_profile_read")
     t0 = time.perf_counter()
     for _ in range(steps):
         loss = train_step(model, x, labels, opt)
@@ -89,12 +91,20 @@ def time_steps(model, x, labels, steps, warmup, profile=False):
     prof = None
     if profile:
         import ctypes
+        L.wt_profile(1)
+        _lib.check(L.wt_profile_read(None, None, None), "wt_profile_read")
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            train_step(model, x, labels, opt)
+        torch.cuda.synchronize()
+        dt_p = (time.perf_counter() - t0) / steps
         ms = (ctypes.c_double * 3)()
         n = (ctypes.c_int64 * 3)()
         w = (ctypes.c_int64 * 3)()
         _lib.check(L.wt_profile_read(ms, n, w), "wt_profile_read")
         L.wt_profile(0)
         prof = {c: (ms[c], n[c], w[c]) for c in range(3)}
+        prof["step_s"] = dt_p
     return dt, float(loss.item()), prof
 
 
@@ -127,30 +137,37 @@ def summarize_pmc(fetch_dir, write_dir, calib_bytes):
             for r in csv.DictReader(open(f)):
                 if r["Counter_Name"] == counter:
                     agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
-        return {k: sum(v) / len(v) for k, v in agg.items()}
+        return agg
 
-    fe, wr = read(fetch_dir, "FETCH_SIZE"), read(write_dir, "WRITE_SIZE")
-    copy = [k for k in fe if "copy" in k.lower() or "elementwise" in k.lower()]
+    fa, wa = read(fetch_dir, "FETCH_SIZE"), read(write_dir, "WRITE_SIZE")
+    fe = {k: sum(v) / len(v) for k, v in fa.items()}
+    wr = {k: sum(v) / len(v) for k, v in wa.items()}
+    copy = [k for k in fa if "copy" in k.lower()]
     if not copy:
         raise SystemExit("no copy kernel in the FETCH_SIZE run (run bench.py with --calib-copy)")
-    ck = max(copy, key=lambda k: fe[k])
-    f_scale = calib_bytes / (fe[ck] * 1024.0)
-    w_scale = calib_bytes / (wr[ck] * 1024.0)
+    # the calibration copies are the largest dispatches of the copy kernel
+    ck = max(copy, key=lambda k: max(fa[k]))
+    cf, cw = max(fa[ck]), max(wa[ck])
+    f_scale = calib_bytes / (cf * 1024.0)
+    w_scale = calib_bytes / (cw * 1024.0)
     meta = json.load(open(os.path.join(fetch_dir, "bench_meta.json")))
     out = {"tuning": meta["tuning"], "calibration": {
-        "kernel": ck, "bytes": calib_bytes, "fetch_kib": fe[ck], "write_kib": wr[ck],
+        "kernel": ck, "bytes": calib_bytes, "fetch_kib": cf, "write_kib": cw,
         "fetch_scale": f_scale, "write_scale": w_scale}, "kernels": {}}
-    lines = [f"copy calibration: {ck}: {calib_bytes} B -> FETCH {fe[ck]:.1f} KiB "
-             f"(scale {f_scale:.4f}), WRITE {wr[ck]:.1f} KiB (scale {w_scale:.4f})"]
+    lines = [f"copy calibration: {ck}: {calib_bytes} B -> FETCH {cf:.1f} KiB "
+             f"(scale {f_scale:.4f}), WRITE {cw:.1f} KiB (scale {w_scale:.4f})"]
     for k in sorted(fe, key=lambda k: -fe[k]):
         if k == ck or k not in wr:
             continue
         name = k.split("(")[0].strip()
         name = name[5:] if name.startswith("void ") else name
         fb, wb = fe[k] * 1024.0 * f_scale, wr[k] * 1024.0 * w_scale
-        out["kernels"][name] = {"fetch_bytes": fb, "write_bytes": wb,
+        out["kernels"][name] = {"dispatches": len(fa[k]), "fetch_kib": fe[k], "write_kib": wr[k],
+                                "fetch_bytes": fb, "write_bytes": wb,
                                 "hbm_bytes_per_launch": fb + wb}
-        lines.append(f"{name}: fetch {fb / 1e6:.3f} MB  write {wb / 1e6:.3f} MB per launch")
+        lines.append(f"{name}: {len(fa[k])} dispatches, FETCH {fe[k]:.1f} KiB, WRITE "
+                     f"{wr[k]:.1f} KiB -> fetch {fb / 1e6:.3f} MB  write {wb / 1e6:.3f} MB "
+                     f"per launch")
     os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
     with open(TRAFFIC_FILE, "w") as f:
         json.dump(out, f, indent=1)
@@ -234,6 +251,7 @@ def main():
     log(f"[bench] fused: {dt * 1e3:.2f} ms/step  loss {loss:.4f}  plan {plan.tuning}")
     value = cells / dt / 1e9
     # dominant kernel class by summed time
+    prof_step_s = prof.pop("step_s")
     cls = max(prof, key=lambda c: prof[c][0])
     ms, nl, work = prof[cls]
     avg_s = ms / max(nl, 1) / 1e3
@@ -265,7 +283,9 @@ def main():
                      "traffic": traffic(kname, plan.tuning),
                      "kernel": kname, "bytes_per_cell_step": BYTES[cls],
                      "cell_steps_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
-                     "launches": nl},
+                     "launches": nl, "events_ms_per_step": round(prof_step_s * 1e3, 3),
+                     "kernel_ms_per_step": {class_name(c, plan.tuning): round(prof[c][0] / args.steps, 3)
+                                            for c in prof}},
         # the whole training step at 28 B per cell-step (12 forward + 16 adjoint)
         "step_roofline": {"achieved": round(28.0 * cells / dt / 1e9, 1), "unit": "GB/s",
                           "frac": round(28.0 * cells / dt / PEAK_HBM, 4)},

@@ -71,7 +71,7 @@ struct AdjArgs {
   const float *A, *Bm, *C2;
   const float *bgeom, *clin, *rho, *rhoc;
   float s2, dtm1, dtm2, t1, t4;
-  float sat_b0, sat_uth, cmin, cmax;
+  float sat_b0, sat_uth, cmin, cmax, c_nl;
   int nlf;
   // adjoint state: lam_{m+1}, lam_{m+2} in; lam_{m_lo}, lam_{m_lo+1} out
   const float *in1, *in2;
@@ -86,7 +86,20 @@ struct AdjArgs {
   float *gsrc;       // (B,T,nsrcslot) lambda_m at source pixels
 };
 
-// Optional phase timestamps (diagnostic builds compiled with
+// Extra buffers of the nonlinear adjoint (one launch per step): products of
+// lambda_m with step m-1 factors handed to the next iterations, and where
+// u_{n0-1} of the current segment lives.
+struct NlAdjBufs {
+  const float *W_in;   // w_m * lambda_{m+1}            (B,Nx,Ny)
+  float *W_out;        // w_{m-1} * lambda_m            (B,Nx,Ny)
+  float *PL;           // P_m lambda_{m+1} in, P_{m-1} lambda_m out (in place)
+  float *AL;           // A Bm lambda of this parity slot (in place)
+  const float *uprev0; // u_{seg_n0 - 1}                (B,Nx,Ny)
+  int rec_ld;          // row stride of the segment record
+  int64_t rec_slice;   // stride of one (sample, step) slice of the record
+};
+
+// Optional phase timestamps// Optional phase timestamps (diagnostic builds compiled with
 // -DWT_PHASE_TIMING): thread 0 of every workgroup records the 100 MHz wall
 // clock at kernel start, after set-up, after the step loop and at the end
 // into phase_ts[(blockIdx.x) * 8 + i].

@@ -285,6 +285,107 @@ __global__ void k_loop_adj_plain(AdjArgs a, const int32_t *__restrict__ spmap,
   }
 }
 
+// One adjoint step of the nonlinear cell (cell.py:94-102 inside the step of
+// cell.py:12-17; what autograd computes chaining cell.py:26-44 with the
+// derivatives of saturable_damping, cell.py:8-9, and of the Kerr-like speed).
+// The step's b and c depend on u_n, so the recurrence is
+//   lambda_m = P_m*lambda_{m+1} + L(w_m*lambda_{m+1}) - A_{m+1}Bm_{m+1}*lambda_{m+2} + g_m
+// with w_m = A_m c_m^2 and the pointwise factor
+//   P_m = s2*A_m + (dv/db)(db/du) + (dv/dc)(dc/du)   at u = u_m,
+//   dv/db = A*dtm1*(u_{m-1} - v_{m+1}),  dv/dc = 2*A*c*lap(u_m),
+//   db/du = -2 rho b0 (u/uth^2) / (1 + (u/uth)^2)^2,  dc/du = 2 rho c_nl u
+// (dc/du and dv/dc vanish where the optional clamp is active).  Iteration m
+// holds lambda_m only in registers: it emits, for the later iterations, the
+// products nb.W_out = w_{m-1}*lambda_m and nb.PL = P_{m-1}*lambda_m (read by
+// iteration m-1) and nb.AL = A_{m-1}Bm_{m-1}*lambda_m (read by iteration
+// m-2, the same parity slot), and accumulates the parameter gradients of
+// step m-1 (u_m = v_m + sources):
+//   gacc   += dv/dc * lambda_m          (dJ/dc_linear)
+//   gacc_b += dv/db * lambda_m          (dJ/db)
+//   gacc_rho += (dv/db * b0/(1+(u/uth)^2) + dv/dc * c_nl u^2) * lambda_m
+// u_{m-1}, u_{m-2} and lap(u_{m-1}) come from the segment record (row stride
+// nb.rec_ld) or, for u_{m-2} at the segment start, from the checkpoint.
+template <bool UNIT_H, int NL>
+__global__ void k_loop_adj_nl_plain(AdjArgs a, const int32_t *__restrict__ spmap, NlAdjBufs nb,
+                                    const float *__restrict__ fbar, int m) {
+  const int Nx = a.Nx, Ny = a.Ny;
+  const int64_t F = (int64_t)Nx * Ny;
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= F * a.B) return;
+  int b = (int)(gid / F);
+  int64_t q = gid - (int64_t)b * F;
+  int i = (int)(q / Ny), j = (int)(q - (int64_t)i * Ny);
+  const float *W = nb.W_in + (int64_t)b * F;
+  float up = i > 0 ? W[q - Ny] : 0.0f;
+  float dn = i < Nx - 1 ? W[q + Ny] : 0.0f;
+  float lf = j > 0 ? W[q - 1] : 0.0f;
+  float rt = j < Ny - 1 ? W[q + 1] : 0.0f;
+  float lapw = lap5<UNIT_H>(up, lf, W[q], rt, dn, a.t1, a.t4);
+  float lam = (nb.PL[gid] + lapw) - nb.AL[gid];
+  int sp = spmap[q];
+  if (sp >= 0 && a.ybar) {
+    const int32_t *e = a.spx + sp * SPX_W;
+    for (int t = e[4]; t < e[5]; ++t) {
+      const int32_t *tp = a.taps + t * TAP_W;
+      int64_t o = ((int64_t)b * a.T + (m - 1)) * a.ntap + tp[3];
+      float gy = a.ybar[o];
+      lam += tp[2] ? 2.0f * a.yraw[o] * gy : gy;
+    }
+  }
+  if (fbar) lam += fbar[((int64_t)b * a.T + (m - 1)) * F + q];
+  // step m-1
+  const int n = m - 1;
+  const int64_t ro = ((int64_t)b * a.seg_K + (n - a.seg_n0)) * nb.rec_slice +
+                     (int64_t)i * nb.rec_ld + j;
+  const float u = a.seg_u[ro];
+  const float lapu = a.seg_lap[ro];
+  const float uprev = (n == a.seg_n0) ? nb.uprev0[gid] : a.seg_u[ro - nb.rec_slice];
+  const float rho = a.rho[q];
+  float bb = a.bgeom[q], cc = a.clin[q], sd = 0.0f, c2raw = 0.0f, dbdu = 0.0f, dcdu = 0.0f;
+  if (NL & 1) {
+    const float r = u / a.sat_uth, d = 1.0f + r * r;
+    sd = a.sat_b0 / d;
+    bb += rho * sd;
+    dbdu = rho * a.sat_b0 * (-2.0f * r / a.sat_uth) / (d * d);
+  }
+  if (NL & 2) {
+    c2raw = a.c_nl * (u * u);
+    cc += rho * c2raw;
+    dcdu = 2.0f * rho * a.c_nl * u;
+  }
+  bool clamped = false;
+  if (NL & 4) {
+    if (cc < a.cmin) { cc = a.cmin; clamped = true; }
+    if (cc > a.cmax) { cc = a.cmax; clamped = true; }
+  }
+  const float bd = bb * a.dtm1;
+  const float A = 1.0f / (a.dtm2 + bd);
+  const float Bm = a.dtm2 - bd;
+  const float v = A * ((a.s2 * u - Bm * uprev) + (cc * cc) * lapu);
+  const float dvdb = A * a.dtm1 * (uprev - v);
+  const float dvdc = clamped ? 0.0f : 2.0f * A * cc * lapu;
+  a.gacc[gid] = fmaf(dvdc, lam, a.gacc[gid]);
+  a.gacc_b[gid] = fmaf(dvdb, lam, a.gacc_b[gid]);
+  a.gacc_rho[gid] = fmaf(dvdb * sd + dvdc * c2raw, lam, a.gacc_rho[gid]);
+  nb.W_out[gid] = A * (cc * cc) * lam;
+  nb.PL[gid] = (a.s2 * A + dvdb * dbdu + dvdc * dcdu) * lam;
+  nb.AL[gid] = (A * Bm) * lam;
+  if (sp >= 0) {
+    const int32_t *e = a.spx + sp * SPX_W;
+    if (e[2] > 0) a.gsrc[((int64_t)b * a.T + (m - 1)) * a.nsrcslot + e[3]] = lam;
+  }
+}
+
+// out[q] = sum_b acc[b, q] (deterministic in-thread order).
+__global__ void k_batch_sum(const float *__restrict__ acc, int B, int64_t F,
+                            float *__restrict__ out) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= F) return;
+  float s = 0.0f;
+  for (int b = 0; b < B; ++b) s += acc[(int64_t)b * F + q];
+  out[q] = s;
+}
+
 // dJ/dc = 2 c A sum_b gacc_b (the batch reduction autograd performs on
 // grad_c, cell.py:36); deterministic in-thread order.
 __global__ void k_finalize_gc(const float *__restrict__ gacc, int B, int64_t F,
@@ -397,6 +498,33 @@ hipError_t launch_adj_plain(bool unit_h, const AdjArgs &a, const int32_t *spmap,
   else
     hipLaunchKernelGGL(k_loop_adj_plain<false>, dim3(nblk(tot, 256)), dim3(256), 0, s, a, spmap,
                        wl1, wl_out, fbar, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_nl_plain(bool unit_h, int nl, const AdjArgs &a, const int32_t *spmap,
+                               const NlAdjBufs &nb, const float *fbar, int m, hipStream_t s) {
+  int64_t tot = (int64_t)a.B * a.Nx * a.Ny;
+  dim3 g(nblk(tot, 256)), bsz(256);
+#define AP(UH, NL) \
+  hipLaunchKernelGGL((k_loop_adj_nl_plain<UH, NL>), g, bsz, 0, s, a, spmap, nb, fbar, m)
+#define AP_NL(UH)                          \
+  switch (nl) {                            \
+    case 1: AP(UH, 1); break;              \
+    case 2: AP(UH, 2); break;              \
+    case 3: AP(UH, 3); break;              \
+    case 5: AP(UH, 5); break;              \
+    case 6: AP(UH, 6); break;              \
+    case 7: AP(UH, 7); break;              \
+    default: return hipErrorInvalidValue;  \
+  }
+  if (unit_h) { AP_NL(true) } else { AP_NL(false) }
+#undef AP_NL
+#undef AP
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_sum(const float *acc, int B, int64_t F, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_sum, dim3(nblk(F, 256)), dim3(256), 0, s, acc, B, F, out);
   return hipGetLastError();
 }
 

@@ -51,6 +51,9 @@ hipError_t launch_fwd_plain(bool unit_h, int nl, bool rec, const FwdArgs &a,
 hipError_t launch_adj_plain(bool unit_h, const AdjArgs &a, const int32_t *spmap,
                             const float *wl1, float *wl_out, const float *fbar, int m,
                             hipStream_t s);
+hipError_t launch_adj_nl_plain(bool unit_h, int nl, const AdjArgs &a, const int32_t *spmap,
+                               const NlAdjBufs &nb, const float *fbar, int m, hipStream_t s);
+hipError_t launch_batch_sum(const float *acc, int B, int64_t F, float *out, hipStream_t s);
 hipError_t launch_finalize_gc(const float *gacc, int B, int64_t F, const float *c,
                               const float *A, float *gc, hipStream_t s);
 hipError_t launch_finalize_gx(const float *gsrc, int B, int T, int nslot, const int32_t *mult,
@@ -851,6 +854,78 @@ int prepare_coef(const Plan &p, const Ws &w, const float *b, const float *c, con
   return WT_OK;
 }
 
+// Reverse sweep of the nonlinear cell: per segment, recompute from the
+// checkpoint recording u_n and lap(u_n), then one k_loop_adj_nl_plain launch
+// per step (the step's coefficients depend on u, so the linear kernels'
+// precomputed a1/a2/w do not apply).
+int adj_nonlinear(const Plan &p, const float *b, const float *c, const float *rho,
+                  const float *x, const float *ybar, const float *fbar, void *saved, void *temp,
+                  float *grad_c, float *grad_b, float *grad_rho, float *grad_x, hipStream_t s) {
+  Ws w = layout(p, saved, temp, true);
+  const int64_t BF = (int64_t)p.B * p.F;
+  const int T = p.T, K = p.K;
+  const size_t segcells = (size_t)p.B * K * p.Nx * (p.Ny + 1);
+  float *st[8];
+  for (int i = 0; i < 8; ++i) st[i] = w.st + i * BF;
+  // PL, AL (two parity slots), w*lambda ping-pong, three accumulators
+  WT_HIP(launch_fill(w.lam, 3 * BF, 0.0f, s));
+  WT_HIP(launch_fill(w.wl, 2 * BF, 0.0f, s));
+  WT_HIP(launch_fill(w.gacc, 3 * BF, 0.0f, s));
+  WT_HIP(launch_fill(w.gsrc, (int64_t)p.B * T * std::max(p.nslot, 1), 0.0f, s));
+
+  FwdArgs fa = base_fwd(p, w, b, c, rho, x);
+  fa.rec_lap = w.seg;
+  fa.rec_u = w.seg + segcells;
+  fa.rec_K = K;
+  const bool cs = p.tiled && p.tun.kind == 0;
+  NlAdjBufs nb{};
+  nb.rec_ld = cs ? p.Ny : p.Ny + 1;
+  nb.rec_slice = (int64_t)p.Nx * nb.rec_ld;
+  AdjArgs aa{};
+  aa.B = p.B; aa.Nx = p.Nx; aa.Ny = p.Ny; aa.T = T;
+  aa.spx = p.d_spx; aa.taps = p.d_taps; aa.ntap = p.ntap;
+  aa.nsrcslot = std::max(p.nslot, 1);
+  aa.bgeom = b; aa.clin = c; aa.rho = rho;
+  aa.s2 = p.s2; aa.dtm1 = p.dtm1; aa.dtm2 = p.dtm2; aa.t1 = p.t1; aa.t4 = p.t4;
+  aa.sat_b0 = p.sat_b0; aa.sat_uth = p.sat_uth; aa.cmin = p.cmin; aa.cmax = p.cmax;
+  aa.c_nl = p.c_nl; aa.nlf = p.nl;
+  aa.ybar = ybar; aa.yraw = w.yraw;
+  aa.seg_lap = fa.rec_lap; aa.seg_u = fa.rec_u; aa.seg_K = K;
+  aa.gacc = w.gacc; aa.gacc_b = w.gacc + BF; aa.gacc_rho = w.gacc + 2 * BF;
+  aa.gsrc = w.gsrc;
+  float *wl1 = w.wl, *wl2 = w.wl + BF;
+  nb.PL = w.lam;
+  for (int sg = p.nseg - 1; sg >= 0; --sg) {
+    const int n0 = sg * K, n1 = std::min(T, n0 + K);
+    const float *ck = w.ck + (size_t)sg * 2 * BF;
+    fa.rec_n0 = n0;
+    float *tmp[4] = {st[0], st[1], st[2], st[3]};
+    int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+    if (rc) return rc;
+    aa.seg_n0 = n0;
+    nb.uprev0 = ck + BF;
+    for (int m = n1; m > n0; --m) {
+      nb.W_in = wl1;
+      nb.W_out = wl2;
+      nb.AL = w.lam + (1 + (m & 1)) * BF;
+      ProfScope ps(2, s, BF);
+      WT_HIP(launch_adj_nl_plain(p.unit_h, p.nl, aa, p.d_spmap, nb, fbar, m, s));
+      std::swap(wl1, wl2);
+    }
+  }
+  const int64_t F = (int64_t)p.F;
+  if (grad_c) WT_HIP(launch_batch_sum(w.gacc, p.B, F, grad_c, s));
+  if (grad_b) WT_HIP(launch_batch_sum(w.gacc + BF, p.B, F, grad_b, s));
+  if (grad_rho) WT_HIP(launch_batch_sum(w.gacc + 2 * BF, p.B, F, grad_rho, s));
+  if (grad_x) {
+    if (p.nslot > 0)
+      WT_HIP(launch_finalize_gx(w.gsrc, p.B, T, p.nslot, p.d_mult, grad_x, s));
+    else
+      WT_HIP(launch_fill(grad_x, (int64_t)p.B * T, 0.0f, s));
+  }
+  return WT_OK;
+}
+
 }  // namespace
 
 int wt_loop_fwd(void *plan, const float *b, const float *c, const float *rho, const float *x,
@@ -905,11 +980,10 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
     set_error("wt_loop_adj: invalid argument");
     return WT_EINVAL;
   }
-  if (p->nl) {
-    set_error("wt_loop_adj: the nonlinear cell adjoint is not available in this build");
-    return WT_EUNSUPPORTED;
-  }
   hipStream_t s = (hipStream_t)stream;
+  if (p->nl)
+    return adj_nonlinear(*p, b, c, rho, x, ybar, fbar, saved, temp, grad_c, grad_b, grad_rho,
+                         grad_x, s);
   Ws w = layout(*p, saved, temp, true);
   const int64_t BF = (int64_t)p->B * p->F;
   const int T = p->T, K = p->K;
@@ -983,10 +1057,6 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
       // the tiled path uses the pair (l3, l4) as outputs; keep the plain
       // rotation consistent with it: l4 unused here
     }
-  }
-  // for the plain path the w*lambda buffer of the tiled path is not used
-  if (!use_tiled) {
-    // nothing
   }
   if (grad_c) {
     // dJ/dc = 2 c A sum_b gacc (the clamp-only case: c is time-invariant and
