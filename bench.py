@@ -1,20 +1,24 @@
-"""Benchmark: one training step of the 150x150, batch-64, 3000-step WaveRNN.
+"""Benchmark of the WaveRNN hot path on one MI355X.
 
-A step is the hot path over one synthetic batch: the fused forward time loop
-(leapfrog + source + 3 intensity probes), the cross-entropy loss head, the
-checkpointed adjoint (recompute every K=64 steps) back to rho through the
-blur/projection, and an Adam update of rho.  Inputs are resident on the GPU
-before the timed region.
+Default workload: one training step of the 150x150, batch-64, 3000-step
+WaveRNN -- the fused forward time loop (leapfrog + source + 3 intensity
+probes), the cross-entropy loss head, the checkpointed adjoint (recompute
+every K=64 steps) back to rho through the blur/projection, and an Adam update
+of rho.  `--workload` selects the other configurations (100x100 batch 4,
+nonlinear 150x150 batch 32, forward-only 1024x1024 with snapshots).  Inputs
+are resident on the GPU before the timed region.
 
 Prints ONE JSON line:
-  value          fwd+adjoint Gcell-steps/s = B*Nx*Ny*T / step time / 1e9
+  value          Gcell-steps/s = B*Nx*Ny*T / step time / 1e9
   roofline       the dominant kernel (by summed time) at 12 B (forward) or
                  16 B (adjoint) of algorithmic HBM traffic per cell-step,
-                 timed with HIP events on its launch stream over the timed
-                 region, against MI355X's 8 TB/s
+                 timed with HIP events on its launch stream over a second
+                 timed region of the same length, against MI355X's 8 TB/s;
+                 `traffic` from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+                 (--calib-copy / --summarize-pmc) when one matches the plan
+  step_roofline  the whole step at its algorithmic bytes per cell-step
   cpu_baseline   the torch-CPU restatement of the reference (oracle/ref_torch)
-                 timed forward+backward on a bounded sample of the same
-                 workload on this host's cores
+                 on a bounded sample of the same workload on this host's cores
   plain          the same step with the one-launch-per-step kernels
 """
 from __future__ import annotations
@@ -33,7 +37,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12
 BYTES = {0: 12.0, 1: 12.0, 2: 16.0}  # algorithmic bytes per cell-step per kernel class
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_train_150_b64.json")
+
+def traffic_file(workload):
+    return os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
 
 
 def class_name(cls, tun):
@@ -51,6 +57,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# Workloads bench.py can time.  The default is the training step the headline
+# metric is quoted on; the others are the remaining configurations of the
+# hot path, timed with the same protocol.
+#   kind     "train": forward + CE loss + checkpointed adjoint + Adam step;
+#            "forward": forward only (no autograd tape), fields snapshotted
+#   bytes    algorithmic HBM bytes per cell-step of one step
+WORKLOADS = {
+    "train_150_b64": dict(
+        kind="train", bytes=28.0,
+        metric="fwd+adjoint Gcell-steps/s (150x150, batch 64, 3000 steps, training step)",
+        desc="150x150 grid incl. 20-cell PML, free-form rho on the 110x110 interior, batch 64, "
+             "3000 steps, 3 intensity probes, CE loss, adjoint with checkpoints every 64 steps, "
+             "Adam"),
+    "nonlinear_150_b32": dict(
+        kind="train", bytes=28.0,
+        metric="fwd+adjoint Gcell-steps/s (150x150, batch 32, 3000 steps, nonlinear cell)",
+        desc="as train_150_b64 at batch 32 with saturable damping b_nl=0.5, I_sat=0.1 and Kerr "
+             "speed c_nl=0.05 clamped to [0.4, 1.1], dt=0.64"),
+    "propagate_100_b4": dict(
+        kind="train", bytes=28.0,
+        metric="fwd+adjoint Gcell-steps/s (100x100, batch 4, 1000 steps)",
+        desc="100x100 grid incl. 20-cell PML, free-form rho on the 60x60 interior, batch 4, "
+             "1000 steps, windowed 500 Hz sine, 3 intensity probes, CE loss"),
+    "scatter_1024": dict(
+        kind="forward", bytes=12.0, field_stride=500,
+        metric="forward Gcell-steps/s (1024x1024, 4000 steps, snapshots every 500)",
+        desc="1024x1024 grid incl. 32-cell PML, c=1 with 20 circular scatterers (c=0.5), "
+             "batch 1, 4000 steps, Gaussian pulse at the centre, field snapshot every 500 steps"),
+}
+
+
 def build(dev, tuning=None, workload="train_150_b64", T=None):
     from wavetorch_amd import workloads
     kw = {} if T is None else {"T": T}
@@ -58,11 +95,17 @@ def build(dev, tuning=None, workload="train_150_b64", T=None):
     model = model.to(dev)
     if tuning is not None:
         model.tuning = tuning
-    return model, x.to(dev), labels.to(dev)
+    fs = WORKLOADS[workload].get("field_stride")
+    if fs:
+        model.field_stride = fs
+    return model, x.to(dev), (labels.to(dev) if labels is not None else None)
 
 
 def train_step(model, x, labels, opt):
     from wavetorch_amd.utils import normalize_power
+    if labels is None:  # forward-only workload: snapshot fields, no tape
+        with torch.no_grad():
+            return model(x, output_fields=True)[0, -1, 0, 0]
     opt.zero_grad(set_to_none=True)
     y = model(x)
     loss = torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels)
@@ -108,12 +151,12 @@ def time_steps(model, x, labels, steps, warmup, profile=False):
     return dt, float(loss.item()), prof
 
 
-def traffic(kname, tun):
+def traffic(workload, kname, tun):
     """Measured HBM bytes per launch of `kname` from the rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary written by --summarize-pmc, when it was taken on the
     same kernel and tuning; else None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(traffic_file(workload)) as f:
             d = json.load(f)
     except (OSError, ValueError):
         return None
@@ -126,7 +169,7 @@ def traffic(kname, tun):
 def summarize_pmc(fetch_dir, write_dir, calib_bytes):
     """Per-kernel average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
     rocprofv3 --pmc runs, each scaled by what the counter reads for the
-    bench's own device copy of `calib_bytes` bytes; writes TRAFFIC_FILE."""
+    bench's own device copy of `calib_bytes` bytes; writes profiles/traffic_<workload>.json."""
     import collections
     import csv
     import glob
@@ -168,48 +211,61 @@ def summarize_pmc(fetch_dir, write_dir, calib_bytes):
         lines.append(f"{name}: {len(fa[k])} dispatches, FETCH {fe[k]:.1f} KiB, WRITE "
                      f"{wr[k]:.1f} KiB -> fetch {fb / 1e6:.3f} MB  write {wb / 1e6:.3f} MB "
                      f"per launch")
-    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
-    with open(TRAFFIC_FILE, "w") as f:
+    path = traffic_file(meta.get("workload", "train_150_b64"))
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print("\n".join(lines))
 
 
-def cpu_baseline(budget_s=15.0):
-    """The reference's op sequence (torch-CPU restatement, fp32) on the same
-    workload with T cut to fit ~budget_s of forward+backward."""
+def cpu_baseline(workload="train_150_b64", budget_s=15.0):
+    """The reference's op sequence (oracle/ref_torch: the torch-CPU restatement
+    of rnn.py/cell.py/source.py/probe.py/geom.py with its autograd tape) on
+    the same workload, with T cut to fit ~budget_s on this host's cores."""
     from oracle import ref_torch as rt
     from wavetorch_amd import workloads
-    model, x, labels = workloads.train_150_b64()
-    g = model.cell.geom
-    ncores = os.cpu_count() or 1
-    torch.set_num_threads(ncores)
-    rho = g.rho.detach().clone().requires_grad_(True)
-    k = g.blur_kernel
-    dt = torch.tensor(0.707)
-    h = torch.tensor(1.0)
-    src = [(torch.tensor(40), torch.tensor(75))]
-    probes = [(torch.tensor(110), torch.tensor(p), True) for p in (55, 75, 95)]
+    spec = WORKLOADS[workload]
+    model, x, labels = getattr(workloads, workload)()
+    g, cell = model.cell.geom, model.cell
+    torch.set_num_threads(os.cpu_count() or 1)
+    train = spec["kind"] == "train"
+    rho = g.rho.detach().clone().requires_grad_(train)
+    src = [(s_.x.clone(), s_.y.clone()) for s_ in model.sources]
+    probes = [(p_.x.clone(), p_.y.clone(), bool(p_.intensity)) for p_ in model.probes]
+    nlkw = {}
+    if float(cell.satdamp_b0) > 0 or float(cell.c_nl) != 0:
+        nlkw = dict(satdamp_b0=cell.satdamp_b0, satdamp_uth=cell.satdamp_uth, c_nl=cell.c_nl,
+                    c_clamp=cell.c_clamp)
 
     def run(T):
-        c = rt.freeform_c(rho, 1.0, 0.5, k, 1, 0.5, 100.0)
-        y = rt.rnn_forward(x[:, :T].contiguous(), c=c, rho=rho, b_geom=g.b, dt=dt, h=h,
-                           sources=src, probes=probes)
-        loss = torch.nn.functional.cross_entropy(rt.normalize_power(y.sum(dim=1)), labels)
-        loss.backward()
+        with torch.set_grad_enabled(train):
+            c = rt.freeform_c(rho, g.c0.item(), g.c1.item(), g.blur_kernel, int(g.blur_N.item()),
+                              g.eta.item(), g.beta.item())
+            y = rt.rnn_forward(x[:, :T].contiguous(), c=c, rho=rho, b_geom=g.b, dt=cell.dt,
+                               h=g.h, sources=src, probes=probes, output_fields=not train,
+                               **nlkw)
+            if train:
+                loss = torch.nn.functional.cross_entropy(rt.normalize_power(y.sum(dim=1)),
+                                                         labels)
+                loss.backward()
 
+    B, Tfull = x.shape
+    Nx, Ny = g.domain_shape
     t0 = time.perf_counter()
-    run(8)
-    est = (time.perf_counter() - t0) / 8
-    T = int(max(16, min(3000, budget_s / max(est, 1e-6))))
+    run(4)
+    est = (time.perf_counter() - t0) / 4
+    T = int(max(8, min(Tfull, budget_s / max(est, 1e-6))))
+    if not train:  # the reference stacks every field: keep the sample within ~2 GB
+        T = min(T, max(8, int(2e9 // (B * Nx * Ny * 4))))
     t0 = time.perf_counter()
     run(T)
     el = time.perf_counter() - t0
-    B, N = x.shape[0], g.domain_shape[0]
-    cells = B * N * N * T
+    cells = B * Nx * Ny * T
+    what = "fwd+autograd bwd through the restated TimeStep.backward" if train else \
+        "forward keeping every field (rnn.py:66-70)"
     return {"value": cells / el / 1e9, "unit": "Gcell-steps/s", "cores": torch.get_num_threads(),
             "kind": "port",
-            "sample": f"batch {B}, {N}x{N}, first {T} of 3000 steps, fwd+autograd bwd through "
-                      f"the restated TimeStep.backward, {el:.1f} s"}
+            "sample": f"batch {B}, {Nx}x{Ny}, first {T} of {Tfull} steps, {what}, {el:.1f} s"}
 
 
 def main():
@@ -219,6 +275,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-kernel comparison")
+    ap.add_argument("--workload", default="train_150_b64", choices=sorted(WORKLOADS),
+                    help="configuration to time (default: the headline training step)")
     ap.add_argument("--tuning", type=str, default=None, help="JSON wt_tuning override")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the timed region, copy MIB MiB device-to-device 3 times (the "
@@ -237,15 +295,16 @@ def main():
         sys.exit(2)
     dev = torch.device("cuda:0")
     torch.cuda.init()
+    wl = WORKLOADS[args.workload]
     tuning = json.loads(args.tuning) if args.tuning else None
     if tuning is None:
         from wavetorch_amd import autotune
-        tuning = autotune.cached("train_150_b64")
-    model, x, labels = build(dev, tuning)
+        tuning = autotune.cached(args.workload)
+    model, x, labels = build(dev, tuning, args.workload)
     B, T = x.shape
-    N = model.cell.geom.domain_shape[0]
-    cells = B * N * N * T
-    log(f"[bench] tuning {tuning}")
+    Nx, Ny = model.cell.geom.domain_shape
+    cells = B * Nx * Ny * T
+    log(f"[bench] {args.workload}: tuning {tuning}")
     dt, loss, prof = time_steps(model, x, labels, args.steps, args.warmup, profile=True)
     plan = next(iter(model._plans.values()))
     log(f"[bench] fused: {dt * 1e3:.2f} ms/step  loss {loss:.4f}  plan {plan.tuning}")
@@ -262,7 +321,7 @@ def main():
             f"{(m_ / max(n_, 1)) * 1e3:.1f} us avg")
     kname = class_name(cls, plan.tuning)
     out = {
-        "metric": "fwd+adjoint Gcell-steps/s (150x150, batch 64, 3000 steps, training step)",
+        "metric": wl["metric"],
         "value": round(value, 3),
         "unit": "Gcell-steps/s",
         "n_gpus": 1,
@@ -273,26 +332,27 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded vowel-like 10 kHz waveforms, random rho)",
-        "config": {"workload": "train_150_b64: 150x150 grid incl. 20-cell PML, free-form rho "
-                               "on the 110x110 interior, batch 64, 3000 steps, 3 intensity "
-                               "probes, CE loss, adjoint with checkpoints every 64 steps, Adam",
-                   "B": B, "Nx": N, "Ny": N, "T": T, "tuning": plan.tuning},
+        "data": "synthetic (seeded inputs and rho, see wavetorch_amd/workloads.py)",
+        "config": {"workload": f"{args.workload}: {wl['desc']}", "B": B, "Nx": Nx, "Ny": Ny,
+                   "T": T, "tuning": plan.tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
                      "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
-                     "traffic": traffic(kname, plan.tuning),
+                     "traffic": traffic(args.workload, kname, plan.tuning),
                      "kernel": kname, "bytes_per_cell_step": BYTES[cls],
                      "cell_steps_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
                      "launches": nl, "events_ms_per_step": round(prof_step_s * 1e3, 3),
-                     "kernel_ms_per_step": {class_name(c, plan.tuning): round(prof[c][0] / args.steps, 3)
-                                            for c in prof}},
-        # the whole training step at 28 B per cell-step (12 forward + 16 adjoint)
-        "step_roofline": {"achieved": round(28.0 * cells / dt / 1e9, 1), "unit": "GB/s",
-                          "frac": round(28.0 * cells / dt / PEAK_HBM, 4)},
+                     "kernel_ms_per_step": {class_name(c, plan.tuning):
+                                            round(prof[c][0] / args.steps, 3)
+                                            for c in prof if prof[c][1]}},
+        # the whole step at its algorithmic bytes per cell-step (28 B = 12 forward
+        # + 16 adjoint for a training step, 12 B forward only)
+        "step_roofline": {"bytes_per_cell_step": wl["bytes"],
+                          "achieved": round(wl["bytes"] * cells / dt / 1e9, 1), "unit": "GB/s",
+                          "frac": round(wl["bytes"] * cells / dt / PEAK_HBM, 4)},
     }
     if args.meta_out:
         with open(args.meta_out, "w") as f:
-            json.dump({"tuning": plan.tuning}, f)
+            json.dump({"tuning": plan.tuning, "workload": args.workload}, f)
     if args.calib_copy > 0:
         n = args.calib_copy * 2 ** 20 // 4
         a = torch.ones(n, device=dev)
@@ -301,8 +361,9 @@ def main():
             bb.copy_(a)
         torch.cuda.synchronize()
         del a, bb
+    del model
     if not args.no_plain:
-        pm, px, pl = build(dev, {"plain": 1})
+        pm, px, pl = build(dev, {"plain": 1}, args.workload)
         pdt, _, _ = time_steps(pm, px, pl, 1, 1)
         log(f"[bench] plain: {pdt * 1e3:.2f} ms/step")
         out["plain_ms_per_step"] = round(pdt * 1e3, 3)
@@ -310,7 +371,7 @@ def main():
         del pm, px, pl
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out), flush=True)
 
 

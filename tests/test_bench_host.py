@@ -1,0 +1,61 @@
+"""CPU checks of bench.py's host logic: the CPU-baseline leg on a tiny time
+sample, the rocprofv3 counter summary with its copy calibration, and the
+single-GPU guard."""
+import csv
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+@pytest.mark.parametrize("wl", ["propagate_100_b4", "scatter_1024"])
+def test_cpu_baseline_runs(wl):
+    r = bench.cpu_baseline(wl, budget_s=0.2)
+    assert r["value"] > 0 and r["kind"] == "port" and r["cores"] >= 1
+    assert "steps" in r["sample"]
+
+
+def _write_counters(d, counter, rows):
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "run_counter_collection.csv"), "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Dispatch_Id", "Kernel_Name", "Counter_Name",
+                                          "Counter_Value"])
+        w.writeheader()
+        for i, (k, v) in enumerate(rows):
+            w.writerow({"Dispatch_Id": i, "Kernel_Name": k, "Counter_Name": counter,
+                        "Counter_Value": v})
+
+
+def test_summarize_pmc_calibrates_on_largest_copy(tmp_path, monkeypatch):
+    cp = "__amd_rocclr_copyBuffer"
+    kn = "void wt::k_adj_cs<1024, 8>(wt::AdjArgs)"
+    # a 1 MiB calibration copy reads as 512 KiB (scale 2) next to small copies
+    _write_counters(tmp_path / "f", "FETCH_SIZE", [(cp, 4.0), (cp, 512.0), (kn, 100.0),
+                                                   (kn, 300.0)])
+    _write_counters(tmp_path / "w", "WRITE_SIZE", [(cp, 2.0), (cp, 256.0), (kn, 50.0),
+                                                   (kn, 50.0)])
+    (tmp_path / "f" / "bench_meta.json").write_text(json.dumps({"tuning": {"a": 1},
+                                                                "workload": "unit_test"}))
+    monkeypatch.setattr(bench, "traffic_file", lambda w: str(tmp_path / f"traffic_{w}.json"))
+    bench.summarize_pmc(str(tmp_path / "f"), str(tmp_path / "w"), 2 ** 20)
+    d = json.loads((tmp_path / "traffic_unit_test.json").read_text())
+    assert d["calibration"]["fetch_scale"] == pytest.approx(2.0)
+    assert d["calibration"]["write_scale"] == pytest.approx(4.0)
+    k = d["kernels"]["wt::k_adj_cs<1024, 8>"]
+    assert k["fetch_bytes"] == pytest.approx(200.0 * 1024 * 2)
+    assert k["write_bytes"] == pytest.approx(50.0 * 1024 * 4)
+    assert bench.traffic("unit_test", "wt::k_adj_cs<1024, 8>", {"a": 1}) == \
+        pytest.approx(k["fetch_bytes"] + k["write_bytes"])
+    assert bench.traffic("unit_test", "wt::k_adj_cs<1024, 8>", {"a": 2}) is None
+
+
+def test_multi_gpu_request_exits_with_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "4"],
+                       capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr

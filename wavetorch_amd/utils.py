@@ -11,8 +11,9 @@ def to_tensor(x, dtype=None):
     dtype = dtype if dtype is not None else torch.get_default_dtype()
     if type(x) is np.ndarray:
         return torch.from_numpy(x).type(dtype=dtype)
-    else:
-        return torch.tensor(x, dtype=dtype)
+    if isinstance(x, torch.Tensor):  # what torch.tensor(x) yields, without its warning
+        return x.detach().clone().to(dtype=dtype)
+    return torch.tensor(x, dtype=dtype)
 
 
 def set_dtype(dtype=None):
