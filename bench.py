@@ -80,6 +80,14 @@ WORKLOADS = {
         metric="fwd+adjoint Gcell-steps/s (100x100, batch 4, 1000 steps)",
         desc="100x100 grid incl. 20-cell PML, free-form rho on the 60x60 interior, batch 4, "
              "1000 steps, windowed 500 Hz sine, 3 intensity probes, CE loss"),
+    "lens_batch_256": dict(
+        kind="batch", bytes=28.0,
+        metric="fwd+adjoint Gcell-steps/s (256 lens instances of uneven size and length, "
+               "one graph)",
+        desc="256 independent lens problems, grid side from {64,96,128,192,256} incl. 16-cell "
+             "PML, 1000-4000 steps, each with its own rho, a line source and a focal intensity "
+             "probe, J = -sum_t u(focal)^2; forward + checkpointed adjoint of every instance as "
+             "one captured HIP graph"),
     "scatter_1024": dict(
         kind="forward", bytes=12.0, field_stride=500,
         metric="forward Gcell-steps/s (1024x1024, 4000 steps, snapshots every 500)",
@@ -268,6 +276,160 @@ def cpu_baseline(workload="train_150_b64", budget_s=15.0):
             "sample": f"batch {B}, {Nx}x{Ny}, first {T} of {Tfull} steps, {what}, {el:.1f} s"}
 
 
+def run_batch(args, dev):
+    """Batched instances: the forward + adjoint of every instance, captured
+    once as a HIP graph and replayed; inputs (c of every instance from its
+    rho, x, dJ/dy) resident in HBM."""
+    import ctypes
+    from wavetorch_amd import _lib, autotune, workloads
+    from wavetorch_amd._loop import Plan, WaveLoop
+    from wavetorch_amd.batch import Instance, InstanceBatch
+    wl = WORKLOADS[args.workload]
+    items = [(m.to(dev), x.to(dev)) for m, x in workloads.lens_batch_256()]
+    with torch.no_grad():
+        cs = [m.cell.geom.c.detach().clone() for m, _ in items]
+    xs = [x for _, x in items]
+    ybars = [-torch.ones((x.shape[0], 1), device=dev) for x in xs]
+    insts = [Instance.from_model(m, x.shape[0]) for m, x in items]
+    tuning = json.loads(args.tuning) if args.tuning else autotune.cached_batch(insts)
+    bt = InstanceBatch(insts, dev, tuning=tuning)
+    bt.set_inputs(cs, xs, ybars)
+    cells = bt.cell_steps()
+    log(f"[bench] {args.workload}: {len(insts)} instances in {len(bt.groups)} groups, "
+        f"{cells:.3g} cell-steps, tuning {bt.tuning[0]}")
+    s = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(s):
+        bt.capture()
+        for _ in range(args.warmup):
+            bt.replay()
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bt.replay()
+        s.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        # per-kernel-class HIP-event timing of one eager run (events cannot sit
+        # inside the graph)
+        L = _lib.lib()
+        L.wt_profile(1)
+        _lib.check(L.wt_profile_read(None, None, None), "wt_profile_read")
+        t0 = time.perf_counter()
+        bt.fwd_adj()
+        s.synchronize()
+        dt_eager = time.perf_counter() - t0
+        ms = (ctypes.c_double * 3)()
+        n = (ctypes.c_int64 * 3)()
+        w = (ctypes.c_int64 * 3)()
+        _lib.check(L.wt_profile_read(ms, n, w), "wt_profile_read")
+        L.wt_profile(0)
+    prof = {c: (ms[c], n[c], w[c]) for c in range(3)}
+    cls = max(prof, key=lambda c: prof[c][0])
+    msc, nl, work = prof[cls]
+    avg_s = msc / max(nl, 1) / 1e3
+    per_launch = work / max(nl, 1)
+    achieved = BYTES[cls] * per_launch / avg_s
+    t0g = bt.tuning[0]
+    names = [f"wt::k_fwd_cs_batch<{t0g['nt']}, {t0g['cpt']}, 0, false>",
+             f"wt::k_fwd_cs_batch<{t0g['nt']}, {t0g['cpt']}, 0, true>",
+             f"wt::k_adj_cs_batch<{t0g['a_nt']}, {t0g['a_cpt']}>"]
+    for c in range(3):
+        log(f"[bench] class {c} {names[c]}: {n[c]} launches, {ms[c]:.2f} ms, "
+            f"{ms[c] / max(n[c], 1) * 1e3:.1f} us avg")
+    log(f"[bench] graph replay {dt * 1e3:.2f} ms, eager {dt_eager * 1e3:.2f} ms")
+    out = {
+        "metric": wl["metric"], "value": round(cells / dt / 1e9, 3), "unit": "Gcell-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded sizes, lengths and rho, see wavetorch_amd/workloads.py)",
+        "config": {"workload": f"{args.workload}: {wl['desc']}", "instances": len(insts),
+                   "groups": [{"side": bt.instances[idx[0]].cfg.Nx, "instances": len(idx),
+                               "strips": bt.tuning[g]["strips"], "tcols": bt.tuning[g]["tcols"]}
+                              for g, idx in enumerate(bt.groups)],
+                   "cell_steps": cells, "tuning": bt.tuning[0]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
+                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM, 4),
+                     "traffic": traffic(args.workload, names[cls], bt.tuning[0]),
+                     "kernel": names[cls], "bytes_per_cell_step": BYTES[cls],
+                     "cell_steps_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
+                     "launches": nl, "eager_ms": round(dt_eager * 1e3, 3)},
+        "step_roofline": {"bytes_per_cell_step": wl["bytes"],
+                          "achieved": round(wl["bytes"] * cells / dt / 1e9, 1), "unit": "GB/s",
+                          "frac": round(wl["bytes"] * cells / dt / PEAK_HBM, 4)},
+    }
+    if args.meta_out:
+        with open(args.meta_out, "w") as f:
+            json.dump({"tuning": bt.tuning[0], "workload": args.workload}, f)
+    if args.calib_copy > 0:
+        nn = args.calib_copy * 2 ** 20 // 4
+        a = torch.ones(nn, device=dev)
+        bb = torch.empty_like(a)
+        for _ in range(3):
+            bb.copy_(a)
+        torch.cuda.synchronize()
+        del a, bb
+    del bt
+    if not args.no_plain:
+        # the plain per-instance path: one plan per instance, one launch per
+        # step, instance after instance (plans made before the clock starts)
+        plans = [Plan(m.loop_config(), 1, x.shape[0], m.cell.geom.cmax, dev, {"plain": 1})
+                 for m, x in items]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for (m, x), c, yb, p in zip(items, cs, ybars, plans):
+            ct = c.clone().requires_grad_(True)
+            y = WaveLoop.apply(x[None], ct, m.cell.geom.b, None, p, False, 1)
+            y.backward(yb[None])
+        torch.cuda.synchronize()
+        pdt = time.perf_counter() - t0
+        log(f"[bench] plain per-instance loop: {pdt * 1e3:.1f} ms")
+        out["plain_ms_per_step"] = round(pdt * 1e3, 3)
+        out["speedup_vs_plain"] = round(pdt / dt, 3)
+        del plans
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline_batch(items, cells)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_batch(items, total_cells, budget_s=15.0):
+    """The reference's op sequence (oracle/ref_torch) on a seeded subset of
+    the instances (each truncated in time), extrapolated to the whole batch
+    by cell-steps."""
+    from oracle import ref_torch as rt
+    torch.set_num_threads(os.cpu_count() or 1)
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(items))
+    done_cells, el = 0, 0.0
+    picked = []
+    for i in order:
+        m, x = items[i]
+        g, cell = m.cell.geom, m.cell
+        N = g.domain_shape[0]
+        T = min(x.shape[0], 200)
+        rho = g.rho.detach().cpu().clone().requires_grad_(True)
+        src = [(s_.x.cpu().clone(), s_.y.cpu().clone()) for s_ in m.sources]
+        probes = [(p_.x.cpu().clone(), p_.y.cpu().clone(), True) for p_ in m.probes]
+        t0 = time.perf_counter()
+        c = rt.freeform_c(rho, g.c0.item(), g.c1.item(), g.blur_kernel.cpu(),
+                          int(g.blur_N.item()), g.eta.item(), g.beta.item())
+        y = rt.rnn_forward(x[:T].cpu()[None], c=c, rho=rho, b_geom=g.b.cpu(), dt=cell.dt.cpu(),
+                           h=g.h.cpu(), sources=src, probes=probes)
+        (-y.sum()).backward()
+        el += time.perf_counter() - t0
+        done_cells += N * N * T
+        picked.append(f"{N}x{N}x{T}")
+        if el > budget_s:
+            break
+    rate = done_cells / el
+    return {"value": rate / 1e9, "unit": "Gcell-steps/s", "cores": torch.get_num_threads(),
+            "kind": "port",
+            "sample": f"{len(picked)} of {len(items)} instances (seeded pick), first 200 steps "
+                      f"each, fwd+autograd bwd, {el:.1f} s; the whole batch "
+                      f"({total_cells:.3g} cell-steps) would take {total_cells / rate:.0f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +458,9 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.init()
     wl = WORKLOADS[args.workload]
+    if wl["kind"] == "batch":
+        run_batch(args, dev)
+        return
     tuning = json.loads(args.tuning) if args.tuning else None
     if tuning is None:
         from wavetorch_amd import autotune

@@ -181,6 +181,57 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho,
                 float *grad_rho, float *grad_x, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Batched instances                                                         */
+/* ------------------------------------------------------------------------ */
+/* Many independent linear WaveRNN problems of uneven grid size and length
+ * (e.g. a set of inverse-design problems, each a WaveRNN of rnn.py:4-72 with
+ * its own c and input length) run as one sequence of launches in which every
+ * launch covers every instance that still has steps to run.  Instances are
+ * passed in groups that share a configuration (grid, PML field b, sources,
+ * probes; at most 8 groups); within a group every instance has its own c and
+ * its own length.  Each instance's result is what wt_loop_fwd/wt_loop_adj
+ * give for it alone (replaces looping rnn.py:21-72 + its autograd backward
+ * over the instances).  The reference runs one configuration at a time. */
+typedef struct wt_batch_group_io {
+  const float *b;     /* (Nx,Ny) PML damping shared by the group */
+  const float *c;     /* (B,Nx,Ny) wave speed of each instance */
+  const float *x;     /* (B,T) input, T = the longest length of the batch */
+  float *y;           /* (B,T,ntap) probe series (forward); entries at or past
+                         an instance's length are unspecified */
+  const float *ybar;  /* (B,T,ntap) dJ/dy (adjoint); entries at or past an
+                         instance's length are ignored */
+  float *grad_c;      /* (B,Nx,Ny) dJ/dc of each instance, or NULL */
+  float *grad_x;      /* (B,T) dJ/dx, or NULL */
+} wt_batch_group_io;
+
+/* ngroups configurations; B[g] instances in group g; lengths = the steps of
+ * every instance, group after group (sum of B entries); cmax[g] for the CFL
+ * check of group g.  tunings: NULL (library choice) or ngroups entries whose
+ * tblock/nt/cpt/a_tblock/a_nt/a_cpt/ck_every are taken from entry 0 (common
+ * to every launch) and strips/tcols (and a_strips/a_tcols) per group.  Only
+ * the linear cell with h = 1 is batched. */
+int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
+                         const int32_t *lengths, const float *cmax,
+                         const wt_tuning *tunings, void **plan_out);
+int wt_batch_plan_tuning(const void *plan, int32_t group, wt_tuning *out);
+/* One saved and one temporary buffer for the whole batch. */
+int wt_batch_workspace_bytes(const void *plan, int64_t *saved, int64_t *temp);
+/* Forward of every instance (probes into io[g].y; checkpoints into saved
+ * when with_grad). */
+int wt_batch_fwd(void *plan, const wt_batch_group_io *io, int32_t with_grad,
+                 void *saved, void *temp, void *stream);
+/* Reverse sweep after wt_batch_fwd(with_grad = 1): io[g].ybar in,
+ * io[g].grad_c / grad_x out. */
+int wt_batch_adj(void *plan, const wt_batch_group_io *io, void *saved,
+                 void *temp, void *stream);
+/* wt_batch_fwd + wt_batch_adj for a loss whose dJ/dy is known before the
+ * forward (a fixed weighting of the probe series), as one call that a graph
+ * can capture whole. */
+int wt_batch_fwd_adj(void *plan, const wt_batch_group_io *io, void *saved,
+                     void *temp, void *stream);
+int wt_batch_plan_destroy(void *plan);
+
+/* ------------------------------------------------------------------------ */
 /* Geometry parameterisation (geom.py:207-233)                               */
 /* ------------------------------------------------------------------------ */
 

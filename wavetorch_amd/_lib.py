@@ -50,6 +50,12 @@ class WtTuning(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class WtBatchIO(ctypes.Structure):
+    """Mirror of ``wt_batch_group_io``."""
+    _fields_ = [("b", c_p), ("c", c_p), ("x", c_p), ("y", c_p), ("ybar", c_p),
+                ("grad_c", c_p), ("grad_x", c_p)]
+
+
 _lock = threading.Lock()
 _lib = None
 
@@ -79,6 +85,17 @@ _SIGS = {
                                           c_f32, c_f32, c_p, c_p, c_p, c_p, c_p]),
     "wt_geom_freeform_c_bwd": (ctypes.c_int, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_i32, c_f32,
                                               c_f32, c_f32, c_f32, c_p, c_p, c_p]),
+    "wt_batch_plan_create": (ctypes.c_int, [c_i32, ctypes.POINTER(WtConfig),
+                                            ctypes.POINTER(c_i32), ctypes.POINTER(c_i32),
+                                            ctypes.POINTER(c_f32), ctypes.POINTER(WtTuning),
+                                            ctypes.POINTER(c_p)]),
+    "wt_batch_plan_tuning": (ctypes.c_int, [c_p, c_i32, ctypes.POINTER(WtTuning)]),
+    "wt_batch_workspace_bytes": (ctypes.c_int, [c_p, ctypes.POINTER(c_i64),
+                                                ctypes.POINTER(c_i64)]),
+    "wt_batch_fwd": (ctypes.c_int, [c_p, ctypes.POINTER(WtBatchIO), c_i32, c_p, c_p, c_p]),
+    "wt_batch_adj": (ctypes.c_int, [c_p, ctypes.POINTER(WtBatchIO), c_p, c_p, c_p]),
+    "wt_batch_fwd_adj": (ctypes.c_int, [c_p, ctypes.POINTER(WtBatchIO), c_p, c_p, c_p]),
+    "wt_batch_plan_destroy": (ctypes.c_int, [c_p]),
     "wt_profile": (ctypes.c_int, [c_i32]),
     "wt_profile_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64),
                                        ctypes.POINTER(c_i64)]),

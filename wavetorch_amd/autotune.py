@@ -130,3 +130,63 @@ def tune(model, x, labels=None, steps=1, verbose=False, max_strips=8, budget_s=1
             d[key(Nx, Ny, B, T, K)] = {"tuning": best, "ms": best_t * 1e3}
             _store(d)
     return best
+
+
+# ------------------------------------------------------------ batched instances
+
+BATCH_CANDIDATES = [
+    {"tblock": 16, "nt": 1024, "cpt": 10, "a_tblock": 12, "a_nt": 1024, "a_cpt": 8},
+    {"tblock": 12, "nt": 1024, "cpt": 8, "a_tblock": 12, "a_nt": 1024, "a_cpt": 8},
+    {"tblock": 24, "nt": 1024, "cpt": 12, "a_tblock": 12, "a_nt": 1024, "a_cpt": 8},
+    {"tblock": 16, "nt": 1024, "cpt": 10, "a_tblock": 8, "a_nt": 1024, "a_cpt": 6},
+    {"tblock": 8, "nt": 1024, "cpt": 8, "a_tblock": 8, "a_nt": 1024, "a_cpt": 6},
+]
+
+
+def batch_key(insts, arch=None):
+    """Cache key of a batch: the sorted (Nx, Ny, length) of its instances."""
+    import hashlib
+    desc = sorted((i.cfg.Nx, i.cfg.Ny, i.length) for i in insts)
+    return f"{arch or _arch()}:batch:{len(insts)}:" + \
+        hashlib.sha1(repr(desc).encode()).hexdigest()[:16]
+
+
+def cached_batch(insts, arch=None):
+    with _lock:
+        e = _load().get(batch_key(insts, arch))
+    return dict(e["tuning"]) if e else None
+
+
+def tune_batch(insts, cs, xs, ybars, device, candidates=None, reps=2, verbose=False):
+    """Times the forward + adjoint of the whole batch (wt_batch_fwd_adj) for
+    each candidate block variant / time-block depth (the per-group tiling
+    follows from each: the largest square tiles that fit) and caches the
+    fastest under batch_key(insts)."""
+    from .batch import InstanceBatch
+    best, best_t = None, float("inf")
+    for cand in candidates or BATCH_CANDIDATES:
+        try:
+            bt = InstanceBatch(insts, device, tuning=cand)
+        except (ValueError, RuntimeError) as e:
+            if verbose:
+                print("skip", cand, e, flush=True)
+            continue
+        bt.set_inputs(cs, xs, ybars)
+        bt.fwd_adj()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bt.fwd_adj()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        if verbose:
+            print(f"{cand} {t * 1e3:.2f} ms", flush=True)
+        if t < best_t:
+            best, best_t = dict(cand), t
+        del bt
+    if best is not None:
+        with _lock:
+            d = _load()
+            d[batch_key(insts)] = {"tuning": best, "ms": best_t * 1e3}
+            _store(d)
+    return best

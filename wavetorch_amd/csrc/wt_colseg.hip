@@ -173,13 +173,19 @@ __device__ __forceinline__ void cs_fwd_rows(const FwdArgs &a, const float *__res
 __device__ unsigned long long *wt_phase_ts = nullptr;
 #endif
 
+// Work item w (sample w / ntiles, tile w % ntiles) of the forward launch
+// described by `a`; shared by the single-configuration kernel and the
+// batched kernel (one launch over several configurations).
 template <int NT, int R, int NL, bool REC>
-__global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
+__device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
   WT_TS(0);
   extern __shared__ float smem[];
   const int tid = threadIdx.x;
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int b = w / a.ntiles, ti = w - (w / a.ntiles) * a.ntiles;
+  // a sample whose own length ends before this launch has nothing to do; a
+  // recompute pass keeps going to the end of the segment holding the
+  // sample's last step, so that the adjoint finds every record it reads
+  if (a.Tlim && (REC ? a.rec_n0 : a.n0) >= a.Tlim[b]) return;
   const int32_t *td = a.tiles + ti * TILE_W;
   const CsGeom g = cs_geom<R>(td, tid);
   const int NBUF = (g.RW + 2) * g.CS;
@@ -206,7 +212,7 @@ __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
       vmask |= 1u << r;
       const int64_t q = (int64_t)(g.R0 + row0 + r) * Ny + (g.C0 + g.c);
       if (NL == 0) {
-        v0 = a.A[q]; v1 = a.Bm[q]; v2 = a.C2[q];
+        v0 = a.A[q]; v1 = a.Bm[q]; v2 = a.C2[q + (int64_t)b * a.c2_bstride];
       } else {
         v0 = a.bgeom[q]; v1 = a.clin[q]; v2 = a.rho[q]; v3 = a.rhoc[q];
       }
@@ -329,6 +335,26 @@ __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
   WT_TS(3);
 }
 
+template <int NT, int R, int NL, bool REC>
+__global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
+  fwd_cs_body<NT, R, NL, REC>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Group of a batched launch that work item w belongs to.
+__device__ __forceinline__ int batch_group(const int32_t (&start)[WT_MAX_GROUPS + 1], int ngrp,
+                                           int w) {
+  int gi = 0;
+  while (gi + 1 < ngrp && w >= start[gi + 1]) ++gi;
+  return gi;
+}
+
+template <int NT, int R, int NL, bool REC>
+__global__ __launch_bounds__(NT) void k_fwd_cs_batch(BatchFwd bl) {
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int gi = batch_group(bl.start, bl.ngrp, w);
+  fwd_cs_body<NT, R, NL, REC>(bl.grp[gi], w - bl.start[gi]);
+}
+
 // ------------------------------------------------------------------ adjoint
 
 // Probe-seed table: at most this many probe pixels per region.
@@ -386,14 +412,15 @@ __device__ __forceinline__ void cs_adj_rows(
   }
 }
 
-// Linear cell.  LDS holds w*lambda.
+// Linear cell.  LDS holds w*lambda.  Work item w of the launch `a`.
 template <int NT, int R>
-__global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
+__device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
   WT_TS(0);
   extern __shared__ float smem[];
   const int tid = threadIdx.x;
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int b = w / a.ntiles, ti = w - (w / a.ntiles) * a.ntiles;
+  // lambda_m = 0 for every m of this launch when the sample ends before it
+  if (a.Tlim && a.m_hi - a.nsteps + 1 > a.Tlim[b]) return;
   const int32_t *td = a.tiles + ti * TILE_W;
   const CsGeom g = cs_geom<R>(td, tid);
   const int NBUF = (g.RW + 2) * g.CS;
@@ -418,7 +445,7 @@ __global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
     float A = 0.f, C2 = 0.f, u1 = 0.f, u2 = 0.f;
     if (g.active && row0 + r < g.RH) {
       const int64_t q = (int64_t)(g.R0 + row0 + r) * Ny + (g.C0 + g.c);
-      A = a.A[q]; C2 = a.C2[q];
+      A = a.A[q]; C2 = a.C2[q + (int64_t)b * a.c2_bstride];
       u1 = a.in1[boff + q];
       u2 = a.in2[boff + q];
     }
@@ -462,7 +489,7 @@ __global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
     const int m = a.m_hi - s;
     const int *e = tb_sp + 6 * seed_px[k];
     float gs = 0.0f;
-    if (a.ybar) {
+    if (a.ybar && (!a.Tlim || m - 1 < a.Tlim[b])) {
       for (int t = e[4]; t < e[5]; ++t) {
         const int32_t *tp = a.taps + t * TAP_W;
         const int64_t o = ((int64_t)b * a.T + (m - 1)) * a.ntap + tp[3];
@@ -563,6 +590,18 @@ __global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
     }
   }
   WT_TS(3);
+}
+
+template <int NT, int R>
+__global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
+  adj_cs_body<NT, R>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+template <int NT, int R>
+__global__ __launch_bounds__(NT) void k_adj_cs_batch(BatchAdj bl) {
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int gi = batch_group(bl.start, bl.ngrp, w);
+  adj_cs_body<NT, R>(bl.grp[gi], w - bl.start[gi]);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -685,6 +724,69 @@ hipError_t launch_adj_cs(int nt, int R, const AdjArgs &a, int RHmax, int RWmax, 
   int nthreads = cs_threads(R, RHmax, RWmax);
   if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt) return hipErrorInvalidConfiguration;
 #define X(A, B) if (nt == A && R == B) return cs_adj_dispatch<A, B>(a, nblocks, nthreads, shmem, s);
+  WT_CS_VARIANTS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+// Batched launches (linear cell): one launch over the work items of every
+// configuration in `bl`; RHmax/RWmax are the largest region over all of them.
+template <int NT, int R>
+static hipError_t cs_fwd_batch_dispatch(bool rec, const BatchFwd &bl, int nblocks, int nthreads,
+                                        size_t shmem, hipStream_t s) {
+#define FB(REC)                                                                       \
+  do {                                                                                \
+    auto kfn = k_fwd_cs_batch<NT, R, 0, REC>;                                         \
+    static bool done = false;                                                         \
+    if (!done) {                                                                      \
+      hipError_t e = cs_allow_lds((const void *)kfn);                                 \
+      if (e != hipSuccess) return e;                                                  \
+      done = true;                                                                    \
+    }                                                                                 \
+    hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, bl);             \
+  } while (0)
+  if (rec) FB(true); else FB(false);
+#undef FB
+  return hipGetLastError();
+}
+
+template <int NT, int R>
+static hipError_t cs_adj_batch_dispatch(const BatchAdj &bl, int nblocks, int nthreads,
+                                        size_t shmem, hipStream_t s) {
+  auto kfn = k_adj_cs_batch<NT, R>;
+  static bool done = false;
+  if (!done) {
+    hipError_t e = cs_allow_lds((const void *)kfn);
+    if (e != hipSuccess) return e;
+    done = true;
+  }
+  hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, bl);
+  return hipGetLastError();
+}
+
+hipError_t launch_fwd_cs_batch(int nt, int R, bool rec, const BatchFwd &bl, int RHmax,
+                               int RWmax, hipStream_t s) {
+  int nblocks = bl.start[bl.ngrp];
+  if (nblocks <= 0) return hipSuccess;
+  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax);
+  int nthreads = cs_threads(R, RHmax, RWmax);
+  if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt) return hipErrorInvalidConfiguration;
+#define X(A, B) \
+  if (nt == A && R == B) return cs_fwd_batch_dispatch<A, B>(rec, bl, nblocks, nthreads, shmem, s);
+  WT_CS_VARIANTS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj_cs_batch(int nt, int R, const BatchAdj &bl, int RHmax, int RWmax,
+                               hipStream_t s) {
+  int nblocks = bl.start[bl.ngrp];
+  if (nblocks <= 0) return hipSuccess;
+  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax);
+  int nthreads = cs_threads(R, RHmax, RWmax);
+  if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt) return hipErrorInvalidConfiguration;
+#define X(A, B) \
+  if (nt == A && R == B) return cs_adj_batch_dispatch<A, B>(bl, nblocks, nthreads, shmem, s);
   WT_CS_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;

@@ -57,6 +57,11 @@ struct FwdArgs {
   // rec[(b*K + (n - rec_n0)) * Nx*(Ny+1) + r*(Ny+1) + c]
   float *rec_lap, *rec_u;
   int rec_n0, rec_K;
+  // batched instances: per-sample step count (a sample's work items skip
+  // launches past its end; NULL = all samples run T steps) and the stride of
+  // the C2 field between samples (0 = one field shared by the batch)
+  const int32_t *Tlim;
+  int c2_bstride;
 };
 
 // Arguments of the fused adjoint kernel (row strips only: C0 = 0, RW = Ny).
@@ -84,7 +89,28 @@ struct AdjArgs {
   float *gacc;       // (B,Nx,Ny) running sum of lap(u_{m-1})*lambda_m (linear)
   float *gacc_b, *gacc_rho;  // nonlinear accumulators (B,Nx,Ny)
   float *gsrc;       // (B,T,nsrcslot) lambda_m at source pixels
+  const int32_t *Tlim;  // per-sample step count, or NULL
+  int c2_bstride;       // C2 stride between samples (0 = shared)
 };
+
+// One launch over several configurations (batched instances of uneven grid
+// size and length): work items [start[g], start[g+1]) run grp[g], each with
+// its own tiles, tables, buffers and sizes; the kernel variant (threads, rows
+// per thread) and the steps of the launch are common.  Passed by value (the
+// whole struct is kernel arguments, well under the 4 KB limit), so a
+// captured graph holds every launch's arguments itself.
+constexpr int WT_MAX_GROUPS = 8;
+struct BatchFwd {
+  int ngrp;
+  int32_t start[WT_MAX_GROUPS + 1];
+  FwdArgs grp[WT_MAX_GROUPS];
+};
+struct BatchAdj {
+  int ngrp;
+  int32_t start[WT_MAX_GROUPS + 1];
+  AdjArgs grp[WT_MAX_GROUPS];
+};
+static_assert(sizeof(BatchAdj) < 4000, "batched launch arguments exceed the kernarg limit");
 
 // Extra buffers of the nonlinear adjoint (one launch per step): products of
 // lambda_m with step m-1 factors handed to the next iterations, and where

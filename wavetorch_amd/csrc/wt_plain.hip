@@ -376,6 +376,24 @@ __global__ void k_loop_adj_nl_plain(AdjArgs a, const int32_t *__restrict__ spmap
   }
 }
 
+// out[i] = in[i]^2 over n values (per-sample C2 of batched instances).
+__global__ void k_square(const float *__restrict__ in, int64_t n, float *__restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = in[i];
+  out[i] = v * v;
+}
+
+// dJ/dc of every sample (batched instances, per-sample c):
+// gc[b, q] = 2 c[b, q] A[q] gacc[b, q].
+__global__ void k_finalize_gc_ps(const float *__restrict__ gacc, int B, int64_t F,
+                                 const float *__restrict__ c, const float *__restrict__ A,
+                                 float *__restrict__ gc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)B * F) return;
+  gc[i] = 2.0f * c[i] * A[i % F] * gacc[i];
+}
+
 // out[q] = sum_b acc[b, q] (deterministic in-thread order).
 __global__ void k_batch_sum(const float *__restrict__ acc, int B, int64_t F,
                             float *__restrict__ out) {
@@ -520,6 +538,20 @@ hipError_t launch_adj_nl_plain(bool unit_h, int nl, const AdjArgs &a, const int3
   if (unit_h) { AP_NL(true) } else { AP_NL(false) }
 #undef AP_NL
 #undef AP
+  return hipGetLastError();
+}
+
+hipError_t launch_square(const float *in, int64_t n, float *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_square, dim3(nblk(n, 256)), dim3(256), 0, s, in, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_gc_ps(const float *gacc, int B, int64_t F, const float *c,
+                                 const float *A, float *gc, hipStream_t s) {
+  int64_t n = (int64_t)B * F;
+  hipLaunchKernelGGL(k_finalize_gc_ps, dim3(nblk(n, 256)), dim3(256), 0, s, gacc, B, F, c, A,
+                     gc);
   return hipGetLastError();
 }
 

@@ -6,6 +6,7 @@
 // adjoint through it, the single-step TimeStep entry points and HIP graph
 // capture.  See include/wavetorch_hip.h for the contract of every export.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -54,6 +55,13 @@ hipError_t launch_adj_plain(bool unit_h, const AdjArgs &a, const int32_t *spmap,
 hipError_t launch_adj_nl_plain(bool unit_h, int nl, const AdjArgs &a, const int32_t *spmap,
                                const NlAdjBufs &nb, const float *fbar, int m, hipStream_t s);
 hipError_t launch_batch_sum(const float *acc, int B, int64_t F, float *out, hipStream_t s);
+hipError_t launch_square(const float *in, int64_t n, float *out, hipStream_t s);
+hipError_t launch_finalize_gc_ps(const float *gacc, int B, int64_t F, const float *c,
+                                 const float *A, float *gc, hipStream_t s);
+hipError_t launch_fwd_cs_batch(int nt, int R, bool rec, const BatchFwd &bl, int RHmax,
+                               int RWmax, hipStream_t s);
+hipError_t launch_adj_cs_batch(int nt, int R, const BatchAdj &bl, int RHmax, int RWmax,
+                               hipStream_t s);
 hipError_t launch_finalize_gc(const float *gacc, int B, int64_t F, const float *c,
                               const float *A, float *gc, hipStream_t s);
 hipError_t launch_finalize_gx(const float *gsrc, int B, int T, int nslot, const int32_t *mult,
@@ -148,6 +156,8 @@ struct Plan {
   int32_t *d_tiles_a = nullptr;
   int ntiles_a = 1, RHmax_a = 0, RWmax_a = 0, tb_a = 1, nt_a = 0, cpt_a = 0;
   bool tiled = false;
+  // batched instances: one c (hence C2) per sample instead of one per plan
+  bool c2ps = false;
   // workspace layout (bytes)
   size_t F = 0;
   size_t saved_coef = 0, saved_ck = 0, saved_yraw = 0, saved_bytes = 0;
@@ -314,7 +324,8 @@ static int plan_layout(Plan &p) {
   size_t BF = (size_t)p.B * F * sizeof(float);
   bool nlc = p.nl != 0;
   // saved: coefficients (3F) or rho*c_nl (F), checkpoints, probe amplitudes
-  p.saved_coef = align_up((nlc ? 1 : 3) * F * sizeof(float));
+  const size_t ncoef = nlc ? 1 : (p.c2ps ? 2 + (size_t)p.B : 3);
+  p.saved_coef = align_up(ncoef * F * sizeof(float));
   p.saved_ck = align_up((size_t)p.nseg * 2 * BF);
   p.saved_yraw = align_up((size_t)p.B * p.T * std::max(p.ntap, 1) * sizeof(float));
   p.saved_bytes = p.saved_coef + p.saved_ck + p.saved_yraw;
@@ -326,7 +337,7 @@ static int plan_layout(Plan &p) {
   size_t segcells = (size_t)p.B * p.K * p.Nx * (p.Ny + 1);
   p.t_seg = align_up((nlc ? 2 : 1) * segcells * sizeof(float));
   p.t_gsrc = align_up((size_t)p.B * p.T * std::max(p.nslot, 1) * sizeof(float));
-  p.t_extra = align_up(3 * F * sizeof(float));  // coefficients when not saving
+  p.t_extra = align_up(ncoef * F * sizeof(float));  // coefficients when not saving
   p.temp_bytes = p.t_state + p.t_lam + p.t_wl + p.t_gacc + p.t_seg + p.t_gsrc + p.t_extra;
   p.temp_bytes_nograd = p.t_state + p.t_extra;
   return WT_OK;
@@ -405,6 +416,8 @@ int wt_step_bwd(const float *b, const float *c, int32_t bc_per_sample, const flo
 }
 
 }  // extern "C"
+
+static int plan_upload(Plan *p);
 
 // Host half of plan creation: validation, special-pixel tables, tiling and
 // variant choice, workspace layout.  No device call except hipGetDevice.
@@ -670,8 +683,19 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
   Plan *p = nullptr;
   int rc0 = plan_build(cfg, B, T, cmax_geom, tuning, &p);
   if (rc0) return rc0;
+  rc0 = plan_upload(p);
+  if (rc0) {
+    free_plan(p);
+    return rc0;
+  }
+  *plan_out = p;
+  return WT_OK;
+}
 
-  // ---- device tables
+}  // extern "C"
+
+// Device copies of a plan's tables (synchronous).
+static int plan_upload(Plan *p) {
   auto up = [&](int32_t **dst, const std::vector<int32_t> &v) -> int {
     size_t n = std::max<size_t>(v.size(), 1) * sizeof(int32_t);
     hipError_t e = hipMalloc((void **)dst, n);
@@ -688,13 +712,12 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
   int rc;
   if ((rc = up(&p->d_spx, p->h_spx)) || (rc = up(&p->d_taps, p->h_taps)) ||
       (rc = up(&p->d_spmap, spmap)) || (rc = up(&p->d_mult, p->h_mult)) ||
-      (rc = up(&p->d_tiles, p->h_tiles)) || (rc = up(&p->d_tiles_a, p->h_tiles_a))) {
-    free_plan(p);
+      (rc = up(&p->d_tiles, p->h_tiles)) || (rc = up(&p->d_tiles_a, p->h_tiles_a)))
     return rc;
-  }
-  *plan_out = p;
   return WT_OK;
 }
+
+extern "C" {
 
 int wt_plan_destroy(void *plan) {
   free_plan((Plan *)plan);
@@ -1073,6 +1096,415 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
       WT_HIP(launch_fill(grad_x, (int64_t)p->B * T, 0.0f, s));
   }
   return WT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+//
+// Many independent linear WaveRNN problems ("instances") of uneven grid size
+// and length.  Instances are grouped by configuration (grid, PML, sources,
+// probes); each group is a plan over its instances with one c per instance
+// and a per-instance step count.  Every launch covers the work items of all
+// groups that still have steps to run (k_fwd_cs_batch / k_adj_cs_batch), so
+// small and large grids fill the GPU together and the whole forward +
+// adjoint is one fixed sequence of launches (capturable in one graph).
+
+namespace {
+
+struct BatchPlan {
+  int ngrp = 0;
+  std::vector<Plan *> g;
+  std::vector<int32_t *> d_Tlim;  // per group, per instance step count (device)
+  std::vector<int> Tg;            // per group longest instance
+  int T = 0, K = 64, nseg = 1;
+  int tb = 16, nt = 1024, R = 10, tb_a = 12, nt_a = 1024, R_a = 8;
+  int RHmax = 0, RWmax = 0, RHmax_a = 0, RWmax_a = 0;
+  std::vector<size_t> soff, toff;
+  size_t saved_bytes = 0, temp_bytes = 0;
+};
+
+void free_batch(BatchPlan *bp) {
+  if (!bp) return;
+  for (Plan *p : bp->g) free_plan(p);
+  for (int32_t *t : bp->d_Tlim)
+    if (t) (void)hipFree(t);
+  delete bp;
+}
+
+AdjArgs base_adj(const Plan &p, const Ws &w, const float *ybar) {
+  AdjArgs aa{};
+  aa.B = p.B; aa.Nx = p.Nx; aa.Ny = p.Ny; aa.T = p.T;
+  aa.ntiles = p.ntiles_a; aa.tiles = p.d_tiles_a; aa.spx = p.d_spx; aa.taps = p.d_taps;
+  aa.ntap = p.ntap; aa.nsrcslot = std::max(p.nslot, 1);
+  aa.A = w.A; aa.Bm = w.Bm; aa.C2 = w.C2;
+  aa.s2 = p.s2; aa.dtm1 = p.dtm1; aa.dtm2 = p.dtm2; aa.t1 = p.t1; aa.t4 = p.t4;
+  aa.ybar = ybar; aa.yraw = w.yraw;
+  aa.seg_lap = w.seg; aa.seg_K = p.K;
+  aa.gacc = w.gacc; aa.gsrc = w.gsrc;
+  return aa;
+}
+
+int batch_forward(const BatchPlan &bp, const wt_batch_group_io *io, bool with_grad, char *saved,
+                  char *temp, hipStream_t s) {
+  const int G = bp.ngrp;
+  std::vector<Ws> W(G);
+  std::vector<std::array<float *, 8>> bufs(G);
+  std::vector<const float *> c1(G), c2(G), i1(G), i2(G);
+  std::vector<float *> o1(G);
+  for (int g = 0; g < G; ++g) {
+    const Plan &p = *bp.g[g];
+    const int64_t F = (int64_t)p.F, BF = (int64_t)p.B * F;
+    W[g] = layout(p, saved ? saved + bp.soff[g] : nullptr, temp + bp.toff[g], with_grad);
+    // A, Bm from the group's b; C2 of every instance from its own c
+    WT_HIP(launch_coef_linear(io[g].b, io[g].c, F, p.dtm1, p.dtm2, W[g].A, W[g].Bm, W[g].C2, s));
+    WT_HIP(launch_square(io[g].c, BF, W[g].C2, s));
+    for (int i = 0; i < 8; ++i) bufs[g][i] = W[g].st + i * BF;
+    float *z = with_grad ? W[g].ck : bufs[g][4];
+    WT_HIP(launch_fill(z, 2 * BF, 0.0f, s));
+    c1[g] = z;
+    c2[g] = z + BF;
+  }
+  for (int sg = 0; sg < bp.nseg; ++sg) {
+    const int n0 = sg * bp.K, n1 = std::min(bp.T, n0 + bp.K);
+    for (int g = 0; g < G; ++g) {
+      const int64_t BF = (int64_t)bp.g[g]->B * bp.g[g]->F;
+      o1[g] = (with_grad && sg + 1 < bp.nseg) ? W[g].ck + (size_t)(sg + 1) * 2 * BF
+                                              : (c1[g] == bufs[g][4] ? bufs[g][6] : bufs[g][4]);
+      i1[g] = c1[g];
+      i2[g] = c2[g];
+    }
+    int pp = 0;
+    for (int n = n0; n < n1;) {
+      const int k = std::min(bp.tb, n1 - n);
+      const bool last = n + k >= n1;
+      BatchFwd bl{};
+      int64_t work = 0;
+      std::vector<float *> out1(G), out2(G);
+      for (int g = 0; g < G; ++g) {
+        out1[g] = last ? o1[g] : bufs[g][2 * pp];
+        out2[g] = last ? o1[g] + (int64_t)bp.g[g]->B * bp.g[g]->F : bufs[g][2 * pp + 1];
+        if (bp.Tg[g] <= n) continue;
+        const Plan &p = *bp.g[g];
+        FwdArgs a = base_fwd(p, W[g], io[g].b, io[g].c, nullptr, io[g].x);
+        a.y = io[g].y;
+        a.yraw = with_grad ? W[g].yraw : nullptr;
+        a.in1 = i1[g]; a.in2 = i2[g]; a.out1 = out1[g]; a.out2 = out2[g];
+        a.n0 = n; a.nsteps = k;
+        a.Tlim = bp.d_Tlim[g];
+        a.c2_bstride = (int)p.F;
+        bl.grp[bl.ngrp] = a;
+        bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + p.B * p.ntiles;
+        ++bl.ngrp;
+        work += (int64_t)p.B * p.F * k;
+      }
+      {
+        ProfScope ps(0, s, work);
+        WT_HIP(launch_fwd_cs_batch(bp.nt, bp.R, false, bl, bp.RHmax, bp.RWmax, s));
+      }
+      for (int g = 0; g < G; ++g) {
+        i1[g] = out1[g];
+        i2[g] = out2[g];
+      }
+      pp ^= 1;
+      n += k;
+    }
+    for (int g = 0; g < G; ++g) {
+      c1[g] = o1[g];
+      c2[g] = o1[g] + (int64_t)bp.g[g]->B * bp.g[g]->F;
+    }
+  }
+  return WT_OK;
+}
+
+int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved, char *temp,
+                  hipStream_t s) {
+  const int G = bp.ngrp;
+  std::vector<Ws> W(G);
+  std::vector<std::array<float *, 4>> lam(G), st(G);
+  for (int g = 0; g < G; ++g) {
+    const Plan &p = *bp.g[g];
+    const int64_t BF = (int64_t)p.B * p.F;
+    W[g] = layout(p, saved + bp.soff[g], temp + bp.toff[g], true);
+    for (int i = 0; i < 4; ++i) {
+      lam[g][i] = W[g].lam + i * BF;
+      st[g][i] = W[g].st + i * BF;
+    }
+    WT_HIP(launch_fill(W[g].lam, 4 * BF, 0.0f, s));  // lambda_{T+1} = lambda_{T+2} = 0
+    WT_HIP(launch_fill(W[g].gacc, BF, 0.0f, s));
+    WT_HIP(launch_fill(W[g].gsrc, (int64_t)p.B * p.T * std::max(p.nslot, 1), 0.0f, s));
+  }
+  for (int sg = bp.nseg - 1; sg >= 0; --sg) {
+    const int n0 = sg * bp.K, n1 = std::min(bp.T, n0 + bp.K);
+    // recompute the segment from its checkpoint, recording lap(u_n)
+    std::vector<const float *> i1(G), i2(G);
+    for (int g = 0; g < G; ++g) {
+      const int64_t BF = (int64_t)bp.g[g]->B * bp.g[g]->F;
+      i1[g] = W[g].ck + (size_t)sg * 2 * BF;
+      i2[g] = i1[g] + BF;
+    }
+    int pp = 0;
+    for (int n = n0; n < n1;) {
+      const int k = std::min(bp.tb, n1 - n);
+      BatchFwd bl{};
+      int64_t work = 0;
+      for (int g = 0; g < G; ++g) {
+        if (bp.Tg[g] <= n0) continue;
+        const Plan &p = *bp.g[g];
+        FwdArgs a = base_fwd(p, W[g], io[g].b, io[g].c, nullptr, io[g].x);
+        a.rec_lap = W[g].seg; a.rec_K = bp.K; a.rec_n0 = n0;
+        a.in1 = i1[g]; a.in2 = i2[g]; a.out1 = st[g][2 * pp]; a.out2 = st[g][2 * pp + 1];
+        a.n0 = n; a.nsteps = k;
+        a.Tlim = bp.d_Tlim[g];
+        a.c2_bstride = (int)p.F;
+        bl.grp[bl.ngrp] = a;
+        bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + p.B * p.ntiles;
+        ++bl.ngrp;
+        work += (int64_t)p.B * p.F * k;
+      }
+      {
+        ProfScope ps(1, s, work);
+        WT_HIP(launch_fwd_cs_batch(bp.nt, bp.R, true, bl, bp.RHmax, bp.RWmax, s));
+      }
+      for (int g = 0; g < G; ++g) {
+        i1[g] = st[g][2 * pp];
+        i2[g] = st[g][2 * pp + 1];
+      }
+      pp ^= 1;
+      n += k;
+    }
+    // adjoint over m = n1 .. n0+1
+    for (int m = n1; m > n0;) {
+      const int k = std::min(bp.tb_a, m - n0);
+      BatchAdj bl{};
+      int64_t work = 0;
+      for (int g = 0; g < G; ++g) {
+        if (bp.Tg[g] <= n0) continue;
+        const Plan &p = *bp.g[g];
+        AdjArgs aa = base_adj(p, W[g], io[g].ybar);
+        aa.in1 = lam[g][0]; aa.in2 = lam[g][1]; aa.out1 = lam[g][2]; aa.out2 = lam[g][3];
+        aa.m_hi = m; aa.nsteps = k; aa.seg_n0 = n0;
+        aa.Tlim = bp.d_Tlim[g];
+        aa.c2_bstride = (int)p.F;
+        bl.grp[bl.ngrp] = aa;
+        bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + p.B * p.ntiles_a;
+        ++bl.ngrp;
+        work += (int64_t)p.B * p.F * k;
+      }
+      {
+        ProfScope ps(2, s, work);
+        WT_HIP(launch_adj_cs_batch(bp.nt_a, bp.R_a, bl, bp.RHmax_a, bp.RWmax_a, s));
+      }
+      for (int g = 0; g < G; ++g) {
+        std::swap(lam[g][0], lam[g][2]);
+        std::swap(lam[g][1], lam[g][3]);
+      }
+      m -= k;
+    }
+  }
+  for (int g = 0; g < G; ++g) {
+    const Plan &p = *bp.g[g];
+    if (io[g].grad_c)
+      WT_HIP(launch_finalize_gc_ps(W[g].gacc, p.B, (int64_t)p.F, io[g].c, W[g].A, io[g].grad_c,
+                                   s));
+    if (io[g].grad_x) {
+      if (p.nslot > 0)
+        WT_HIP(launch_finalize_gx(W[g].gsrc, p.B, p.T, p.nslot, p.d_mult, io[g].grad_x, s));
+      else
+        WT_HIP(launch_fill(io[g].grad_x, (int64_t)p.B * p.T, 0.0f, s));
+    }
+  }
+  return WT_OK;
+}
+
+bool batch_io_ok(const BatchPlan *bp, const wt_batch_group_io *io, bool adj) {
+  if (!bp || !io) return false;
+  for (int g = 0; g < bp->ngrp; ++g) {
+    if (!io[g].b || !io[g].c || !io[g].x) return false;
+    if (!adj && !io[g].y) return false;
+    if (adj && !io[g].ybar) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
+                         const int32_t *lengths, const float *cmax, const wt_tuning *tunings,
+                         void **plan_out) {
+  if (!plan_out || !cfgs || !B || !lengths || !cmax || ngroups <= 0 ||
+      ngroups > WT_MAX_GROUPS) {
+    set_error("wt_batch_plan_create: invalid argument (1 <= ngroups <= %d)", WT_MAX_GROUPS);
+    return WT_EINVAL;
+  }
+  *plan_out = nullptr;
+  BatchPlan *bp = new BatchPlan();
+  bp->ngrp = ngroups;
+  const wt_tuning *t0 = tunings;
+  if (t0) {
+    bp->tb = t0->tblock > 0 ? t0->tblock : bp->tb;
+    bp->nt = t0->nt > 0 ? t0->nt : bp->nt;
+    bp->R = t0->cpt > 0 ? t0->cpt : bp->R;
+    bp->tb_a = t0->a_tblock > 0 ? t0->a_tblock : bp->tb_a;
+    bp->nt_a = t0->a_nt > 0 ? t0->a_nt : bp->nt_a;
+    bp->R_a = t0->a_cpt > 0 ? t0->a_cpt : bp->R_a;
+    bp->K = t0->ck_every > 0 ? t0->ck_every : bp->K;
+  }
+  // lengths: concatenated per group
+  int off = 0;
+  for (int g = 0; g < ngroups; ++g) {
+    if (B[g] <= 0) {
+      set_error("wt_batch_plan_create: group %d has no instances", g);
+      free_batch(bp);
+      return WT_EINVAL;
+    }
+    int tg = 0;
+    for (int i = 0; i < B[g]; ++i) {
+      if (lengths[off + i] <= 0) {
+        set_error("wt_batch_plan_create: instance %d of group %d has length %d", i, g,
+                  lengths[off + i]);
+        free_batch(bp);
+        return WT_EINVAL;
+      }
+      tg = std::max(tg, (int)lengths[off + i]);
+    }
+    bp->Tg.push_back(tg);
+    bp->T = std::max(bp->T, tg);
+    off += B[g];
+  }
+  bp->K = std::min(bp->K, bp->T);
+  bp->nseg = (bp->T + bp->K - 1) / bp->K;
+  bp->tb = std::min(bp->tb, bp->K);
+  bp->tb_a = std::min(bp->tb_a, bp->K);
+  off = 0;
+  size_t soff = 0, toff = 0;
+  for (int g = 0; g < ngroups; ++g) {
+    const wt_config &cf = cfgs[g];
+    if (cf.sat_b0 > 0 || cf.c_nl != 0 || cf.c_clamp || cf.h != 1.0f) {
+      set_error("wt_batch_plan_create: group %d: batched instances take the linear cell with "
+                "h = 1", g);
+      free_batch(bp);
+      return WT_EUNSUPPORTED;
+    }
+    // the largest square tiles whose regions fit both block variants
+    // (unless the group's tuning names its tiling)
+    Plan *p = nullptr;
+    int rc = WT_EUNSUPPORTED;
+    const bool given = tunings && tunings[g].strips > 0;
+    const int gmax = given ? 1 : std::max(cf.Nx, cf.Ny);
+    for (int q = 1; q <= gmax && rc; ++q) {
+      wt_tuning tu{};
+      tu.kind = 0;
+      tu.strips = given ? tunings[g].strips : q;
+      tu.tcols = given ? std::max(tunings[g].tcols, 1) : q;
+      tu.a_strips = given && tunings[g].a_strips > 0 ? tunings[g].a_strips : tu.strips;
+      tu.a_tcols = given && tunings[g].a_tcols > 0 ? tunings[g].a_tcols : tu.tcols;
+      tu.tblock = bp->tb; tu.nt = bp->nt; tu.cpt = bp->R;
+      tu.a_tblock = bp->tb_a; tu.a_nt = bp->nt_a; tu.a_cpt = bp->R_a;
+      tu.ck_every = bp->K;
+      rc = plan_build(&cf, B[g], bp->T, cmax[g], &tu, &p);
+      if (rc == WT_EINVAL || rc == WT_ECFL) break;
+    }
+    if (rc) {
+      if (rc == WT_EUNSUPPORTED)
+        set_error("wt_batch_plan_create: group %d (%dx%d) fits no tiling of the block variants "
+                  "(%d threads x %d rows, %d x %d)", g, cf.Nx, cf.Ny, bp->nt, bp->R, bp->nt_a,
+                  bp->R_a);
+      free_batch(bp);
+      return rc;
+    }
+    p->c2ps = true;
+    plan_layout(*p);
+    bp->g.push_back(p);
+    if ((rc = plan_upload(p))) {
+      free_batch(bp);
+      return rc;
+    }
+    int32_t *d = nullptr;
+    hipError_t e = hipMalloc((void **)&d, (size_t)B[g] * sizeof(int32_t));
+    if (e == hipSuccess)
+      e = hipMemcpy(d, lengths + off, (size_t)B[g] * sizeof(int32_t), hipMemcpyHostToDevice);
+    bp->d_Tlim.push_back(d);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "wt_batch_plan_create: lengths table");
+      free_batch(bp);
+      return rc;
+    }
+    off += B[g];
+    bp->RHmax = std::max(bp->RHmax, p->RHmax);
+    bp->RWmax = std::max(bp->RWmax, p->RWmax);
+    bp->RHmax_a = std::max(bp->RHmax_a, p->RHmax_a);
+    bp->RWmax_a = std::max(bp->RWmax_a, p->RWmax_a);
+    bp->soff.push_back(soff);
+    bp->toff.push_back(toff);
+    soff += align_up(p->saved_bytes);
+    toff += align_up(p->temp_bytes);
+  }
+  bp->saved_bytes = soff;
+  bp->temp_bytes = toff;
+  *plan_out = bp;
+  return WT_OK;
+}
+
+int wt_batch_plan_destroy(void *plan) {
+  free_batch((BatchPlan *)plan);
+  return WT_OK;
+}
+
+int wt_batch_plan_tuning(const void *plan, int32_t group, wt_tuning *out) {
+  const BatchPlan *bp = (const BatchPlan *)plan;
+  if (!bp || !out || group < 0 || group >= bp->ngrp) {
+    set_error("wt_batch_plan_tuning: invalid argument");
+    return WT_EINVAL;
+  }
+  *out = bp->g[group]->tun;
+  return WT_OK;
+}
+
+int wt_batch_workspace_bytes(const void *plan, int64_t *saved, int64_t *temp) {
+  const BatchPlan *bp = (const BatchPlan *)plan;
+  if (!bp) {
+    set_error("wt_batch_workspace_bytes: null plan");
+    return WT_EINVAL;
+  }
+  if (saved) *saved = (int64_t)bp->saved_bytes;
+  if (temp) *temp = (int64_t)bp->temp_bytes;
+  return WT_OK;
+}
+
+int wt_batch_fwd(void *plan, const wt_batch_group_io *io, int32_t with_grad, void *saved,
+                 void *temp, void *stream) {
+  const BatchPlan *bp = (const BatchPlan *)plan;
+  if (!batch_io_ok(bp, io, false) || !temp || (with_grad && !saved)) {
+    set_error("wt_batch_fwd: invalid argument");
+    return WT_EINVAL;
+  }
+  return batch_forward(*bp, io, with_grad != 0, (char *)saved, (char *)temp,
+                       (hipStream_t)stream);
+}
+
+int wt_batch_adj(void *plan, const wt_batch_group_io *io, void *saved, void *temp,
+                 void *stream) {
+  const BatchPlan *bp = (const BatchPlan *)plan;
+  if (!batch_io_ok(bp, io, true) || !saved || !temp) {
+    set_error("wt_batch_adj: invalid argument");
+    return WT_EINVAL;
+  }
+  return batch_adjoint(*bp, io, (char *)saved, (char *)temp, (hipStream_t)stream);
+}
+
+int wt_batch_fwd_adj(void *plan, const wt_batch_group_io *io, void *saved, void *temp,
+                     void *stream) {
+  const BatchPlan *bp = (const BatchPlan *)plan;
+  if (!batch_io_ok(bp, io, false) || !batch_io_ok(bp, io, true) || !saved || !temp) {
+    set_error("wt_batch_fwd_adj: invalid argument");
+    return WT_EINVAL;
+  }
+  int rc = batch_forward(*bp, io, true, (char *)saved, (char *)temp, (hipStream_t)stream);
+  if (rc) return rc;
+  return batch_adjoint(*bp, io, (char *)saved, (char *)temp, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------- profiling

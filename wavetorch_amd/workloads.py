@@ -14,6 +14,10 @@ signals instead of recordings (no datasets are available offline):
                          and a Kerr-like speed
 * ``scatter_1024``       1024x1024 grid, 20 circular scatterers, one
                          Gaussian-pulse source, forward only
+* ``lens_batch_256``     256 independent lens-style problems, grid side drawn
+                         from {64, 96, 128, 192, 256} (16-cell PML), length
+                         drawn from [1000, 4000], each with its own rho, a
+                         line source and one focal intensity probe
 """
 from __future__ import annotations
 
@@ -26,7 +30,7 @@ from .cell import WaveCell
 from .geom import WaveGeometryFreeForm
 from .probe import WaveIntensityProbe
 from .rnn import WaveRNN
-from .source import WaveSource
+from .source import WaveLineSource, WaveSource
 
 
 def vowel_like(B: int, T: int, sr: float = 10000.0, seed: int = 2, snr_db: float = 30.0):
@@ -123,3 +127,32 @@ def scatter_1024(T=4000, N=1024, pml=32, nscat=20, seed=1000):
     n = np.arange(T)
     x = np.exp(-((n - 60) / 20.0) ** 2).astype(np.float32)[None]
     return model, torch.from_numpy(x), None
+
+
+def lens_instance(N, T, seed, pml=16):
+    """One lens-style inverse-design problem (after study/optimize_lens.py:
+    14-70): free-form rho on an N x N grid with a `pml`-cell PML, a horizontal
+    line source across the middle half near the top edge carrying
+    sin(2 pi 0.1 n), and one focal intensity probe on the centre line three
+    quarters down.  Returns (model, x (T,))."""
+    g = torch.Generator().manual_seed(seed)
+    rho = torch.rand(N, N, generator=g)
+    geom = WaveGeometryFreeForm((N, N), 1.0, c0=1.0, c1=0.5, abs_N=pml, abs_sig=1.0, abs_p=2.0,
+                                eta=0.5, beta=100.0, rho=rho, blur_radius=1, blur_N=1)
+    cell = WaveCell(0.707, geom)
+    r = pml + 4
+    src = WaveLineSource(r, N // 4, r, 3 * N // 4 - 1)
+    probe = WaveIntensityProbe(N - pml - N // 4, N // 2)
+    model = WaveRNN(cell, src, [probe])
+    x = np.sin(2 * np.pi * 0.1 * np.arange(T)).astype(np.float32)
+    return model, torch.from_numpy(x)
+
+
+def lens_batch_256(n=256, seed=4000, sides=(64, 96, 128, 192, 256), tmin=1000, tmax=4000):
+    """`n` lens problems with seeded sides and lengths; returns a list of
+    (model, x) -- the loss of each is J = -sum_n y[n] (maximise the focal
+    intensity), so dJ/dy = -1 over its length."""
+    rng = np.random.default_rng(seed + 3)
+    N = rng.choice(np.asarray(sides), n)
+    T = rng.integers(tmin, tmax + 1, n)
+    return [lens_instance(int(N[i]), int(T[i]), seed + 10 + i) for i in range(n)]
