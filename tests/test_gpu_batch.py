@@ -79,7 +79,7 @@ def test_batch_graph_replay(cuda):
     results = []
     with torch.cuda.stream(s):
         for rep in range(3):
-            scale = 1.0 + 0.1 * rep
+            scale = 1.0 - 0.05 * rep  # c <= 1 keeps dt = 0.707 inside the CFL bound
             cs = [torch.from_numpy(it[1]).to(cuda) * scale if rep else
                   torch.from_numpy(it[1]).to(cuda) for it in items]
             bt.set_inputs(cs, xs, ybars)

@@ -132,10 +132,13 @@ def test_full_batch_blocked_equals_plain(wl, cuda):
 
 def test_nonlinear_150_vs_oracle(cuda):
     """The nonlinear 150x150 configuration (saturable damping + clamped Kerr
-    speed, dt 0.64) at batch 2 over 1500 steps: forward bitwise, dJ/dc,
-    dJ/drho and dJ/dx within 1e-4 of the fp64 oracle."""
+    speed, dt 0.64) at batch 2 over 600 steps: forward bitwise, dJ/dc,
+    dJ/drho and dJ/dx within 1e-4 of the fp64 oracle.  (Past ~900 steps the
+    exact gradient of this configuration moves by more than 1e-4 under a 1e-7
+    relative change of its inputs -- test_oracle.py pins that -- so no fp32
+    computation can match it there.)"""
     import math
-    model, x, _ = workloads.nonlinear_150_b32(B=2, T=1500)
+    model, x, _ = workloads.nonlinear_150_b32(B=2, T=600)
     model, cfg, c, b, x = _loop_inputs(model, x, cuda)
     B, T = x.shape
     rho = model.cell.geom.rho.detach().clone()

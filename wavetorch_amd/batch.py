@@ -119,6 +119,9 @@ class InstanceBatch:
             self.io[g] = _lib.WtBatchIO(*[bf[k].data_ptr() for k in
                                           ("b", "c", "x", "y", "ybar", "grad_c", "grad_x")])
         self._graph = None
+        # the buffers were zeroed on the current stream; callers may run on
+        # another one
+        torch.cuda.synchronize(self.device)
 
     def __del__(self):
         try:
