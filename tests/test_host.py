@@ -54,18 +54,26 @@ def c2_cfg(N=150):
 
 
 def test_resolve_training_shape_heuristic():
-    """Batch 64 on 150x150: 2x2 tiles per sample (256 workgroups), 16-step
-    forward blocks at 10 rows per thread, 12-step adjoint blocks at 8."""
+    """Batch 64 on 150x150: 2x2 tiles per sample (256 workgroups), the
+    deepest blocks that fit 1024 threads -- forward 32 steps at 12 rows per
+    thread, adjoint 13 steps at 8."""
     t, saved, temp = resolve(c2_cfg(), 64, 3000, 1.0)
     assert (t["plain"], t["kind"], t["strips"], t["tcols"]) == (0, 0, 2, 2)
-    assert (t["tblock"], t["nt"], t["cpt"]) == (16, 1024, 10)
+    assert (t["tblock"], t["nt"], t["cpt"]) == (32, 1024, 12)
     assert (t["a_strips"], t["a_tcols"], t["a_tblock"], t["a_nt"], t["a_cpt"]) == \
-        (2, 2, 12, 1024, 8)
+        (2, 2, 13, 1024, 8)
     assert t["ck_every"] == 64
     F = 150 * 150
     # checkpoints (u_n, u_{n-1}) for ceil(T/K) segments dominate the saved bytes
     assert saved >= math.ceil(3000 / 64) * 2 * 64 * F * 4
     assert saved < math.ceil(3000 / 64) * 2 * 64 * F * 4 + 8 * 2 ** 20
+
+
+def test_resolve_single_tile_runs_whole_segments():
+    """One tile per sample has no halo: one launch per checkpoint segment."""
+    cfg = LoopConfig(Nx=40, Ny=36, dt=0.707, h=1.0, src_px=((9, 18, 0),), taps=((30, 12, 1),))
+    t, _, _ = resolve(cfg, 2, 120, 1.0, {"ck_every": 32})
+    assert (t["strips"], t["tcols"], t["tblock"], t["a_tblock"]) == (1, 1, 32, 32)
 
 
 def test_resolve_keeps_explicit_tuning():

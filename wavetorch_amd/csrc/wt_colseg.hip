@@ -366,11 +366,14 @@ constexpr int CS_MAX_SEEDS = 16;
 // into SGn first, so their latency is covered by a whole step.  The record
 // is read through a range-checked buffer based at the tile's first output
 // row (zero outside the output rows).  a2 = A*Bm is formed as a1 - 1 (with
-// a1 = A*s2 = 2/(1+b*dt), a2 = (1-b*dt)/(1+b*dt)).
+// a1 = A*s2 = 2/(1+b*dt), a2 = (1-b*dt)/(1+b*dt)).  The thread's own w*lambda
+// values (up/centre/down neighbours) are re-formed as KW*Lc -- the same fp32
+// product the previous step stored to LDS -- instead of being kept in R more
+// registers.
 template <int R, bool SP>
 __device__ __forceinline__ void cs_adj_rows(
     const float *__restrict__ P, float *__restrict__ Q, float (&Lc)[R], float (&Ln)[R],
-    float (&WL)[R], const float (&KA1)[R], const float (&KW)[R], float (&G)[R],
+    const float (&KA1)[R], const float (&KW)[R], float (&G)[R],
     const float (&SGc)[R], float (&SGn)[R], int base, int CS, __amdgpu_buffer_rsrc_t rsn,
     bool prefetch, uint32_t voff, int Ny, uint32_t omask, int sr0, int sr1, float gs0,
     float gs1, int sl0, int sl1, float *__restrict__ gsrc_row) {
@@ -380,10 +383,10 @@ __device__ __forceinline__ void cs_adj_rows(
     for (int r = 0; r < R; ++r) SGn[r] = cs_bload(rsn, voff + (uint32_t)(r * Ny * 4));
   }
   float upv = P[base - 1];
+  float ct = KW[0] * Lc[0];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const float ct = WL[r];
-    const float dn = (r < R - 1) ? WL[r + 1] : P[base + R];
+    const float dn = (r < R - 1) ? KW[r + 1] * Lc[r + 1] : P[base + R];
     const float lf = P[lfb + r], rt = P[rtb + r];
     float lapw = upv + lf;
     lapw = __builtin_fmaf(-4.0f, ct, lapw);
@@ -396,10 +399,9 @@ __device__ __forceinline__ void cs_adj_rows(
       if (r == sr1) lam += gs1;
     }
     upv = ct;
+    ct = dn;
     Ln[r] = lam;
-    const float wl = KW[r] * lam;
-    WL[r] = wl;
-    Q[base + r] = wl;
+    Q[base + r] = KW[r] * lam;
     G[r] = __builtin_fmaf(SGc[r], lam, G[r]);
   }
   if (SP) {
@@ -436,7 +438,7 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
   __shared__ int tb_sp[CS_MAX_TABLE * 6];  // x, y, mult, slot, tap_lo, tap_hi
   const int nsp = min(td[9] - td[8], CS_MAX_TABLE);
 
-  float WL[R], L1[R], L2[R], KA1[R], KW[R], G[R], SG0[R], SG1[R];
+  float L1[R], L2[R], KA1[R], KW[R], G[R], SG0[R], SG1[R];
   const int row0 = g.k * R;
   // 1) all global loads first
   float KC2[R];
@@ -468,7 +470,7 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
     const float A = KA1[r];
     KA1[r] = A * s2; KW[r] = A * KC2[r];
     G[r] = 0.0f; SG1[r] = 0.0f;
-    WL[r] = KW[r] * L1[r];
+
   }
   __syncthreads();
   if (tid == 0) {
@@ -480,7 +482,7 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
   }
   if (g.active) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf0[g.base + r] = WL[r];
+    for (int r = 0; r < R; ++r) buf0[g.base + r] = KW[r] * L1[r];
   }
   __syncthreads();
   // seeds[k][s] = sum over taps of pixel k of dy/du * ybar at step m = m_hi - s
@@ -554,10 +556,10 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
       if (wave_sp) {
         const float gs0 = sd0 >= 0 ? seeds[sd0 * a.nsteps + s] : 0.0f;
         const float gs1 = sd1 >= 0 ? seeds[sd1 * a.nsteps + s] : 0.0f;
-        cs_adj_rows<R, true>(P, Q, Lc, Ln, WL, KA1, KW, G, SGc, SGn, g.base, g.CS, rsn, prefetch,
+        cs_adj_rows<R, true>(P, Q, Lc, Ln, KA1, KW, G, SGc, SGn, g.base, g.CS, rsn, prefetch,
                              voff, Ny, omask, sr0, sr1, gs0, gs1, sl0, sl1, gsrc_row);
       } else {
-        cs_adj_rows<R, false>(P, Q, Lc, Ln, WL, KA1, KW, G, SGc, SGn, g.base, g.CS, rsn,
+        cs_adj_rows<R, false>(P, Q, Lc, Ln, KA1, KW, G, SGc, SGn, g.base, g.CS, rsn,
                               prefetch, voff, Ny, omask, -1, -1, 0.f, 0.f, -1, -1, gsrc_row);
       }
     }

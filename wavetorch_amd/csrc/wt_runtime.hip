@@ -283,6 +283,19 @@ static bool tiles_ok_cs(const Plan &p, int nt, int R) {
 static const int kCsVariants[][2] = {{1024, 4},  {1024, 6},  {1024, 8},  {1024, 10},
                                      {1024, 12}, {512, 16},  {512, 20},  {512, 24}};
 
+// A 1024-thread variant with at most maxR rows per thread that holds the
+// current tiling (the smallest such R).
+static bool fit_cs(const Plan &p, int maxR, int &R) {
+  for (auto &v : kCsVariants) {
+    if (v[0] != 1024 || v[1] > maxR) continue;
+    if (tiles_ok_cs(p, v[0], v[1])) {
+      R = v[1];
+      return true;
+    }
+  }
+  return false;
+}
+
 // Smallest R whose block fits (fewest rows per thread -> most threads).
 static bool pick_variant_cs(Plan &p, int &nt, int &R) {
   for (auto &v : kCsVariants) {
@@ -586,6 +599,26 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
       }
       tiled = false;
     } else {
+      if (tun.kind == 0 && !(tuning && tuning->tblock > 0) && !(tuning && tuning->nt > 0)) {
+        // deepest forward block (fewest launches per checkpoint segment)
+        // whose regions still fit 1024 threads at <= 12 rows per thread,
+        // while a tile's own rows and columns stay at least twice its halo
+        const int own = std::min(p->Nx / strips, p->Ny / tcols);
+        // a single tile per sample has no halo: one launch per segment
+        const int whole = (strips == 1 && tcols == 1) ? std::min(p->K, 256) : 0;
+        for (int d : {whole, 32, 24}) {
+          if (d <= tb || d > p->K || (d != whole && own < 2 * d)) continue;
+          build_tiles(*p, strips, tcols, d);
+          int r = 0;
+          if (fit_cs(*p, 12, r)) {
+            tb = d;
+            cpt = r;
+            nt = 1024;
+            break;
+          }
+        }
+        build_tiles(*p, strips, tcols, tb);
+      }
       tun.strips = strips;
       tun.tcols = tcols;
       tun.tblock = tb;
@@ -608,11 +641,26 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
     int fnt = p->ntiles, fRH = p->RHmax, fRW = p->RWmax, fLW = p->LWmax;
     int as = tun.a_strips > 0 ? tun.a_strips : tun.strips;
     int ac = tun.a_tcols > 0 ? tun.a_tcols : (tun.tcols > 0 ? tun.tcols : 1);
-    // the adjoint holds three fields per row in registers: a shallower
-    // default block keeps its tile within 8 rows per thread
+    // the adjoint holds three fields per row in registers: by default the
+    // deepest block whose tiles fit 1024 threads at <= 8 rows per thread
     int atb = tun.a_tblock > 0 ? tun.a_tblock
                                : (tuning && tuning->tblock > 0 ? tun.tblock
                                                                : std::min(tun.tblock, 12));
+    if (tun.a_tblock <= 0 && !(tuning && tuning->tblock > 0) && tun.a_nt <= 0) {
+      const int as0 = tun.a_strips > 0 ? tun.a_strips : tun.strips;
+      const int ac0 = tun.a_tcols > 0 ? tun.a_tcols : (tun.tcols > 0 ? tun.tcols : 1);
+      const int own = std::min(p->Nx / as0, p->Ny / ac0);
+      const int whole = (as0 == 1 && ac0 == 1) ? std::min(p->K, 256) : 0;
+      for (int d : {whole, 16, 15, 14, 13, 12, 10, 8, 6, 4, 2, 1}) {
+        if (d <= 0 || d > p->K || (d != whole && d > 1 && own < 2 * d)) continue;
+        build_tiles(*p, as0, ac0, d);
+        int r = 0;
+        if (fit_cs(*p, 8, r)) {
+          atb = d;
+          break;
+        }
+      }
+    }
     if (atb > p->K) atb = p->K;
     if (atb > 256) atb = 256;
     int ant = tun.a_nt > 0 ? tun.a_nt : tun.nt;
