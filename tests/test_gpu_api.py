@@ -1,0 +1,173 @@
+"""GPU parity of the public module API (the drop-in surface of
+wavetorch/__init__.py:2-10) against the reference's own golden vectors and the
+op-for-op torch restatement of the reference (oracle/ref_torch.py).
+
+* TimeStep (cell.py:20-44): forward bitwise equal to the reference's
+  _time_step output recorded in tests/golden; backward within 1e-5 of the
+  reference's TimeStep.backward, with needs_input_grad gating and per-sample
+  material fields.
+* WaveGeometryFreeForm.c (geom.py:207-233) and its backward vs torch autograd
+  through the restated blur and projection.
+* WaveRNN.forward (rnn.py:21-72) end to end through the module API: probe
+  series (intensity and amplitude, single- and multi-pixel probes, several
+  sources, a line source) bitwise equal to the restated reference loop given
+  the same c, and within rel-L2 1e-5 through the whole rho -> c -> loop chain
+  (the GPU's tanh projection and torch's CPU tanh agree to rounding only);
+  output_fields; dJ/drho through the whole chain within 1e-4 of torch
+  autograd through the restatement.
+* WaveCell.forward (cell.py:79-107), the single-step module call.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ref_torch as rt
+from tests.helpers import rel
+from wavetorch_amd import (TimeStep, WaveCell, WaveGeometryFreeForm, WaveIntensityProbe,
+                           WaveLineSource, WaveProbe, WaveRNN, WaveSource)
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                            "reference_vectors.npz"))
+
+
+def test_timestep_golden(cuda):
+    g = {k: torch.from_numpy(GOLD[k]) for k in GOLD.files}
+    b, c = g["step_b"].to(cuda).requires_grad_(True), g["step_c"].to(cuda).requires_grad_(True)
+    y1 = g["step_y1"].to(cuda).requires_grad_(True)
+    y2 = g["step_y2"].to(cuda).requires_grad_(True)
+    y = TimeStep.apply(b, c, y1, y2, g["step_dt"], torch.tensor(1.0))
+    assert torch.equal(y.detach().cpu(), g["step_y"])
+    y.backward(g["step_g"].to(cuda))
+    for t, k in ((b, "step_gb"), (c, "step_gc"), (y1, "step_gy1"), (y2, "step_gy2")):
+        assert rel(t.grad.cpu().numpy(), GOLD[k]) < 1e-5, k
+
+
+def test_timestep_gating_and_per_sample_material(cuda):
+    rng = np.random.default_rng(1)
+    B, Nx, Ny = 3, 24, 21
+    f = lambda *s: torch.from_numpy(rng.uniform(0.1, 1.0, s).astype(np.float32))  # noqa: E731
+    b, c, y1, y2, g = f(B, Nx, Ny), f(B, Nx, Ny), f(B, Nx, Ny), f(B, Nx, Ny), f(B, Nx, Ny)
+    dt, h = torch.tensor(0.5), torch.tensor(1.0)
+    # reference: the restated TimeStep on the CPU
+    tc = [t.clone().requires_grad_(True) for t in (b, c, y1, y2)]
+    yr = rt.TimeStepRef.apply(*tc, dt, h)
+    yr.backward(g)
+    tg = [t.to(cuda).requires_grad_(i != 2) for i, t in enumerate((b, c, y1, y2))]
+    y = TimeStep.apply(*tg, dt, h)
+    assert torch.equal(y.detach().cpu(), yr.detach())
+    y.backward(g.to(cuda))
+    assert tg[2].grad is None  # needs_input_grad gating (cell.py:37)
+    for i in (0, 1, 3):
+        assert rel(tg[i].grad.cpu().numpy(), tc[i].grad.numpy()) < 1e-5, i
+
+
+def test_freeform_c_and_backward(cuda):
+    rng = np.random.default_rng(2)
+    N = 40
+    rho0 = torch.from_numpy(rng.uniform(0, 1, (N, N)).astype(np.float32))
+    geom = WaveGeometryFreeForm((N, N), 1.0, c0=1.0, c1=0.5, abs_N=6, abs_sig=1.0, abs_p=2,
+                                rho=rho0, blur_radius=1, blur_N=2, beta=4.0).to(cuda)
+    rho_ref = geom.rho.detach().cpu().clone().requires_grad_(True)
+    cr = rt.freeform_c(rho_ref, 1.0, 0.5, geom.blur_kernel.cpu(), 2, 0.5, 4.0)
+    c = geom.c
+    assert rel(c.detach().cpu().numpy(), cr.detach().numpy()) < 1e-6
+    w = torch.from_numpy(rng.standard_normal((N, N)).astype(np.float32))
+    (cr * w).sum().backward()
+    (c * w.to(cuda)).sum().backward()
+    assert rel(geom.rho.grad.cpu().numpy(), rho_ref.grad.numpy()) < 1e-5
+
+
+def _model(N=48, pml=6, seed=3):
+    rng = np.random.default_rng(seed)
+    rho = torch.from_numpy(rng.uniform(0, 1, (N, N)).astype(np.float32))
+    geom = WaveGeometryFreeForm((N, N), 1.0, c0=1.0, c1=0.5, abs_N=pml, abs_sig=1.0, abs_p=2,
+                                rho=rho, beta=8.0)
+    cell = WaveCell(0.707, geom)
+    sources = [WaveSource(pml + 4, N // 2), WaveLineSource(pml + 8, N // 4, pml + 8, N // 2)]
+    probes = [WaveIntensityProbe(N - pml - 6, N // 3), WaveProbe(N - pml - 6, 2 * N // 3),
+              WaveIntensityProbe(N - pml - 9, N // 2)]
+    return WaveRNN(cell, sources, probes)
+
+
+def _ref_loop(model, x, output_fields=False, c=None):
+    """The restated reference loop; `c` replaces the restated rho -> c chain
+    (to compare the loop alone given the same c)."""
+    g, cell = model.cell.geom, model.cell
+    rho = g.rho.detach().cpu().clone().requires_grad_(True)
+    if c is None:
+        c = rt.freeform_c(rho, g.c0.item(), g.c1.item(), g.blur_kernel.cpu(),
+                          int(g.blur_N.item()), g.eta.item(), g.beta.item())
+    src = [(s.x.cpu(), s.y.cpu()) for s in model.sources]
+    probes = [(p.x.cpu(), p.y.cpu(), bool(p.intensity)) for p in model.probes]
+    y = rt.rnn_forward(x.cpu(), c=c, rho=rho, b_geom=g.b.cpu(), dt=cell.dt.cpu(), h=g.h.cpu(),
+                       sources=src, probes=probes, output_fields=output_fields)
+    return y, rho
+
+
+def test_wavernn_end_to_end(cuda):
+    model = _model()
+    T, B = 160, 1  # WaveLineSource broadcasts only at batch 1 in the reference
+    n = np.arange(T)
+    x = torch.from_numpy((np.sin(2 * np.pi * 0.05 * n) * np.exp(-((n - 50) / 20.0) ** 2))
+                         .astype(np.float32))[None].repeat(B, 1)
+    yr, rho_ref = _ref_loop(model, x)
+    w = torch.from_numpy(np.random.default_rng(4).standard_normal(tuple(yr.shape))
+                         .astype(np.float32))
+    (yr * w).sum().backward()
+    model = model.to(cuda)
+    y = model(x.to(cuda))
+    assert y.shape == yr.shape
+    # the loop given the same c is bitwise the reference's; c itself (tanh
+    # projection on the GPU vs torch's CPU tanh) agrees to rounding, so the
+    # whole chain is compared at the forward tolerance
+    with torch.no_grad():
+        cg = model.cell.geom.c.cpu()
+    yc, _ = _ref_loop(model, x, c=cg)
+    assert torch.equal(y.detach().cpu(), yc.detach())
+    assert rel(y.detach().cpu().numpy(), yr.detach().numpy()) < 1e-5
+    (y * w.to(cuda)).sum().backward()
+    assert rel(model.cell.geom.rho.grad.cpu().numpy(), rho_ref.grad.numpy()) < 1e-4
+    # every field (rnn.py:66-70)
+    with torch.no_grad():
+        f = model(x.to(cuda)[:, :40], output_fields=True)
+        fr, _ = _ref_loop(model, x[:, :40], output_fields=True, c=cg)
+    assert torch.equal(f.cpu(), fr.detach())
+
+
+def test_wavernn_multipixel_probe_batch(cuda):
+    N, pml = 40, 5
+    geom = WaveGeometryFreeForm((N, N), 1.0, c0=1.0, c1=0.5, abs_N=pml, rho="rand")
+    model = WaveRNN(WaveCell(0.6, geom), WaveSource(10, 20),
+                    [WaveIntensityProbe([30, 31, 32], [10, 20, 25]),
+                     WaveIntensityProbe([30, 30, 33], [12, 22, 28])])
+    x = torch.from_numpy(np.random.default_rng(5).standard_normal((3, 90)).astype(np.float32))
+    model = model.to(cuda)
+    with torch.no_grad():
+        cg = model.cell.geom.c.cpu()
+    yr, _ = _ref_loop(model, x, c=cg)
+    y = model(x.to(cuda))
+    assert y.shape == yr.shape == (3, 90, 3, 2)
+    assert torch.equal(y.detach().cpu(), yr.detach())
+
+
+def test_wavecell_single_step(cuda):
+    model = _model()
+    cell = model.cell
+    g = cell.geom
+    rng = np.random.default_rng(6)
+    h1 = torch.from_numpy(rng.standard_normal((2, 48, 48)).astype(np.float32))
+    h2 = torch.from_numpy(rng.standard_normal((2, 48, 48)).astype(np.float32))
+    with torch.no_grad():
+        c = rt.freeform_c(g.rho.detach(), 1.0, 0.5, g.blur_kernel, 1, 0.5, 8.0)
+        yr, _ = rt.cell_forward(h1, h2, c, g.rho, b_geom=g.b, dt=cell.dt, h=g.h,
+                                satdamp_b0=cell.satdamp_b0, satdamp_uth=cell.satdamp_uth,
+                                c_nl=cell.c_nl)
+    cell = cell.to(cuda)
+    with torch.no_grad():
+        y, h = cell(h1.to(cuda), h2.to(cuda), c.to(cuda), cell.geom.rho)
+    assert torch.equal(y.cpu(), yr)
+    assert torch.equal(h.cpu(), h1)
