@@ -1,17 +1,18 @@
 """Autotuner for the temporally blocked kernels.
 
-Searches row strips per sample x time-block depth x (threads, cells per
-thread) for one problem shape by timing the forward and the adjoint on the
-GPU, and stores the winner in an on-disk cache keyed by
-(Nx, Ny, B, T, K, gfx arch).  ``WaveRNN.tuning`` takes the winning
-``wt_tuning`` fields; without a cached entry the library's heuristic applies.
+``tune`` searches tiles per sample x time-block depth x rows per thread (for
+the forward and the adjoint separately) for one problem shape by timing whole
+training steps on the GPU, starting from the library's own plan; ``tune_batch``
+does the same for batched instances.  Winners go to an on-disk cache keyed by
+(Nx, Ny, B, T, K, gfx arch) or by the batch's composition; ``WaveRNN.tuning``
+takes the winning ``wt_tuning`` fields, and without a cached entry the
+library's heuristic applies.
 
 The cache file is ``wavetorch_amd/tuning_cache.json`` (JSON, written
 atomically; a corrupt file is ignored and rewritten).
 """
 from __future__ import annotations
 
-import itertools
 import json
 import os
 import threading
@@ -22,10 +23,6 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(_HERE, "tuning_cache.json")
 _lock = threading.Lock()
-
-VARIANTS = [(1024, 8), (1024, 12), (1024, 16), (512, 16), (512, 24), (512, 32), (256, 32),
-            (256, 48)]
-
 
 def _arch():
     try:
@@ -70,76 +67,105 @@ def cached(workload: str):
     return lookup(*shapes[workload])
 
 
-def candidates(Nx, B, max_strips=8):
-    out = []
-    for strips, tb, (nt, cpt) in itertools.product(range(1, max_strips + 1), (4, 8, 12, 16, 24, 32),
-                                                   VARIANTS):
-        if strips == 1 and tb != 32:
-            continue  # whole-domain regions have no halo: depth only sets the launch count
-        out.append({"strips": strips, "tblock": tb, "nt": nt, "cpt": cpt})
-    return out
+# Options of each tuning field the search moves through (column-segment
+# family; rows per thread `cpt` must be a compiled variant).
+AXES = {
+    "tiles": (1, 2, 3, 4),           # g x g tiles per sample (forward)
+    "tblock": (8, 12, 16, 20, 24, 32),
+    "cpt": (6, 8, 10, 12),
+    "a_tiles": (1, 2, 3, 4),         # g x g tiles per sample (adjoint)
+    "a_tblock": (6, 8, 10, 12, 13, 16),
+    "a_cpt": (4, 6, 8),
+}
 
 
-def tune(model, x, labels=None, steps=1, verbose=False, max_strips=8, budget_s=120.0):
-    """Times fwd+adjoint for every feasible candidate on ``model``/``x`` (on
-    the GPU) and caches the fastest; returns it."""
-    from ._loop import Plan  # noqa: F401  (plan creation validates feasibility)
+def _to_tuning(st):
+    return {"kind": 0, "strips": st["tiles"], "tcols": st["tiles"], "tblock": st["tblock"],
+            "nt": 1024, "cpt": st["cpt"], "a_strips": st["a_tiles"], "a_tcols": st["a_tiles"],
+            "a_tblock": st["a_tblock"], "a_nt": 1024, "a_cpt": st["a_cpt"]}
+
+
+def _from_tuning(t):
+    return {"tiles": t["strips"], "tblock": t["tblock"], "cpt": t["cpt"],
+            "a_tiles": t["a_strips"], "a_tblock": t["a_tblock"], "a_cpt": t["a_cpt"]}
+
+
+def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2):
+    """Coordinate search over the tuning fields of AXES for ``model`` on
+    batch ``x``, timing a full training step (forward, loss, checkpointed
+    adjoint) on the GPU.  Starts from the library's own choice, tries every
+    option of one field at a time (the rest fixed), keeps any improvement and
+    repeats for `rounds` passes or until `budget_s`; caches the winner under
+    key(Nx, Ny, B, T, K) and returns it."""
     from .utils import normalize_power
     B, T = x.shape
     Nx, Ny = model.cell.geom.domain_shape
     K = int(model.checkpoint_every)
-    best, best_t = None, float("inf")
     t_start = time.perf_counter()
-    for cand in candidates(Nx, B, max_strips):
-        if time.perf_counter() - t_start > budget_s:
-            break
-        model.tuning = dict(cand)
+
+    def step():
+        model.zero_grad(set_to_none=True)
+        y = model(x)
+        if labels is None:
+            loss = -y.sum()
+        else:
+            loss = torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels)
+        loss.backward()
+
+    def measure(tun):
+        model.tuning = dict(tun)
         try:
-            model.plan(B, T, x.device)
-        except (ValueError, RuntimeError):
-            continue
-        try:
-            def step():
-                model.zero_grad(set_to_none=True)
-                y = model(x)
-                if labels is None:
-                    loss = -y.sum()
-                else:
-                    loss = torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)),
-                                                             labels)
-                loss.backward()
+            plan = model.plan(B, T, x.device)
             step()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(steps):
                 step()
             torch.cuda.synchronize()
-            t = (time.perf_counter() - t0) / steps
-        except RuntimeError as e:  # an infeasible variant reported by the library
+        except (ValueError, RuntimeError) as e:  # a tiling the variants cannot hold
             if verbose:
-                print("skip", cand, e)
-            continue
+                print("skip", tun, str(e)[:120], flush=True)
+            return None, float("inf")
+        t = (time.perf_counter() - t0) / steps
         if verbose:
-            print(f"{cand} {t * 1e3:.2f} ms", flush=True)
-        if t < best_t:
-            best, best_t = dict(cand), t
+            print(f"{ {k: v for k, v in plan.tuning.items() if v} } {t * 1e3:.2f} ms", flush=True)
+        return plan.tuning, t
+
+    model.tuning = None
+    base, best_t = measure({})
+    if base is None:
+        return None
+    state = _from_tuning(base)
+    tried = {tuple(sorted(state.items()))}
+    for _ in range(rounds):
+        improved = False
+        for field, options in AXES.items():
+            for v in options:
+                cand = dict(state, **{field: v})
+                key_ = tuple(sorted(cand.items()))
+                if key_ in tried or time.perf_counter() - t_start > budget_s:
+                    continue
+                tried.add(key_)
+                _, t = measure(_to_tuning(cand))
+                if t < best_t * 0.995:
+                    state, best_t, improved = cand, t, True
+        if not improved:
+            break
+    best = _to_tuning(state)
     model.tuning = best
-    if best is not None:
-        with _lock:
-            d = _load()
-            d[key(Nx, Ny, B, T, K)] = {"tuning": best, "ms": best_t * 1e3}
-            _store(d)
+    with _lock:
+        d = _load()
+        d[key(Nx, Ny, B, T, K)] = {"tuning": best, "ms": best_t * 1e3}
+        _store(d)
     return best
 
 
 # ------------------------------------------------------------ batched instances
 
 BATCH_CANDIDATES = [
-    {"tblock": 16, "nt": 1024, "cpt": 10, "a_tblock": 12, "a_nt": 1024, "a_cpt": 8},
-    {"tblock": 12, "nt": 1024, "cpt": 8, "a_tblock": 12, "a_nt": 1024, "a_cpt": 8},
-    {"tblock": 24, "nt": 1024, "cpt": 12, "a_tblock": 12, "a_nt": 1024, "a_cpt": 8},
-    {"tblock": 16, "nt": 1024, "cpt": 10, "a_tblock": 8, "a_nt": 1024, "a_cpt": 6},
-    {"tblock": 8, "nt": 1024, "cpt": 8, "a_tblock": 8, "a_nt": 1024, "a_cpt": 6},
+    {"tblock": tb, "nt": 1024, "cpt": r, "a_tblock": atb, "a_nt": 1024, "a_cpt": ar}
+    for tb, r in ((16, 10), (16, 12), (24, 10), (24, 12), (32, 12))
+    for atb, ar in ((12, 8), (13, 8), (16, 8))
 ]
 
 

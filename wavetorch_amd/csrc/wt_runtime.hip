@@ -601,7 +601,7 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
     } else {
       if (tun.kind == 0 && !(tuning && tuning->tblock > 0) && !(tuning && tuning->nt > 0)) {
         // deepest forward block (fewest launches per checkpoint segment)
-        // whose regions still fit 1024 threads at <= 12 rows per thread,
+        // whose regions still fit 1024 threads at <= 10 rows per thread,
         // while a tile's own rows and columns stay at least twice its halo
         const int own = std::min(p->Nx / strips, p->Ny / tcols);
         // a single tile per sample has no halo: one launch per segment
@@ -610,7 +610,7 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
           if (d <= tb || d > p->K || (d != whole && own < 2 * d)) continue;
           build_tiles(*p, strips, tcols, d);
           int r = 0;
-          if (fit_cs(*p, 12, r)) {
+          if (fit_cs(*p, 10, r)) {
             tb = d;
             cpt = r;
             nt = 1024;
