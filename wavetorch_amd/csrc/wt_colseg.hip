@@ -606,6 +606,256 @@ __global__ __launch_bounds__(NT) void k_adj_cs_batch(BatchAdj bl) {
   adj_cs_body<NT, R>(bl.grp[gi], w - bl.start[gi]);
 }
 
+// ------------------------------------------------------------------ nonlinear adjoint
+//
+// Temporally blocked adjoint of the nonlinear cell (cell.py:94-102 inside
+// cell.py:12-17; the per-step form is k_loop_adj_nl_plain in wt_plain.hip,
+// whose formulas this follows term by term).  The step's b and c depend on
+// u, so
+//   lambda_m = P_m lambda_{m+1} + L(w_m lambda_{m+1}) - a2_{m+1} lambda_{m+2} + g_m
+// with w_m = A_m c_m^2, a2_m = A_m Bm_m and P_m the pointwise factor.  Each
+// iteration m forms lambda_m, then the coefficients of step m-1 from
+// u_{m-1}, u_{m-2} and lap(u_{m-1}) (segment record), accumulates that step's
+// parameter gradients and carries w_{m-1}, P_{m-1}, a2_{m-1} in registers to
+// the next iterations.  Between launches the carried coefficients travel
+// through global memory with the two lambdas (NlCarry), so a launch may start
+// at any step.  The halo cells also need the record (their coefficients
+// drive lambda inside the cone), so the record is read over the whole region.
+template <int NT, int R, bool GB>
+__global__ __launch_bounds__(NT) void k_adj_cs_nl(AdjArgs a, NlCarry cy) {
+  extern __shared__ float smem[];
+  const int tid = threadIdx.x;
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = w / a.ntiles, ti = w - (w / a.ntiles) * a.ntiles;
+  const int32_t *td = a.tiles + ti * TILE_W;
+  const CsGeom g = cs_geom<R>(td, tid);
+  const int NBUF = (g.RW + 2) * g.CS;
+  float *buf0 = smem, *buf1 = smem + NBUF;
+  float *seeds = smem + 2 * NBUF;
+  // material (b_geom, c_linear, rho, 0) of every region cell, LDS index as
+  // the field buffers, one 16-byte read per cell and step
+  float4 *mat = (float4 *)(smem + ((2 * NBUF + CS_MAX_SEEDS * CS_MAX_TBLOCK + 3) & ~3));
+  __shared__ int seed_px[CS_MAX_SEEDS];
+  __shared__ int nseeds;
+  __shared__ int tb_sp[CS_MAX_TABLE * 6];
+  const int Ny = a.Ny;
+  const int64_t F = (int64_t)a.Nx * Ny;
+  const int64_t boff = (int64_t)b * F;
+  const int nsp = min(td[9] - td[8], CS_MAX_TABLE);
+  const int row0 = g.k * R;
+  const int col = g.C0 + g.c;
+
+  float Lc[R], Ln[R], Wm[R], Pm[R], A2a[R], A2b[R], U1[R], Gc[R], Gr[R], Gb[R];
+  uint32_t vmask = 0;
+  // byte offset of this thread's first cell within one (Nx,Ny) field
+  const uint32_t voff = (uint32_t)(((g.R0 + row0) * Ny + col) * 4);
+  // 1) global loads: lambdas, carried coefficients, u_{m_hi - 1}, material
+  {
+    const __amdgpu_buffer_rsrc_t ru = cs_rsrc(
+        a.seg_u + ((int64_t)b * a.seg_K + (a.m_hi - 1 - a.seg_n0)) * F, F * 4);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      float l1 = 0.f, l2 = 0.f, wv = 0.f, pv = 0.f, xa = 0.f, xb = 0.f, uu = 0.f;
+      float4 mt = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (g.active && row0 + r < g.RH) {
+        vmask |= 1u << r;
+        const int64_t q = (int64_t)(g.R0 + row0 + r) * Ny + col;
+        l1 = a.in1[boff + q]; l2 = a.in2[boff + q];
+        wv = cy.w_in[boff + q]; pv = cy.p_in[boff + q];
+        xa = cy.a2a_in[boff + q]; xb = cy.a2b_in[boff + q];
+        uu = cs_bload(ru, voff + (uint32_t)(r * Ny * 4));
+        mt = make_float4(a.bgeom[q], a.clin[q], a.rho[q], 0.f);
+      }
+      Lc[r] = l1; Ln[r] = l2; Wm[r] = wv; Pm[r] = pv; A2a[r] = xa; A2b[r] = xb; U1[r] = uu;
+      Gc[r] = 0.f; Gr[r] = 0.f; Gb[r] = 0.f;
+      if (g.active) mat[g.base + r] = mt;
+    }
+  }
+  int e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0;
+  if (tid < nsp) {
+    const int32_t *e = a.spx + (td[8] + tid) * SPX_W;
+    e0 = e[0]; e1 = e[1]; e2 = e[2]; e3 = e[3]; e4 = e[4]; e5 = e[5];
+  }
+  // 2) LDS: w_m * lambda_{m+1} of the region, tables, probe seeds
+  for (int i = tid; i < 2 * NBUF; i += blockDim.x) smem[i] = 0.0f;
+  if (tid < nsp) {
+    int *t = tb_sp + 6 * tid;
+    t[0] = e0; t[1] = e1; t[2] = e2; t[3] = e3; t[4] = e4; t[5] = e5;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int n = 0;
+    for (int kk = 0; kk < nsp && n < CS_MAX_SEEDS; ++kk)
+      if (tb_sp[6 * kk + 5] > tb_sp[6 * kk + 4]) seed_px[n++] = kk;
+    nseeds = n;
+  }
+  if (g.active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf0[g.base + r] = Wm[r] * Lc[r];
+  }
+  __syncthreads();
+  for (int i = tid; i < nseeds * a.nsteps; i += blockDim.x) {
+    const int k = i / a.nsteps, s = i - (i / a.nsteps) * a.nsteps;
+    const int m = a.m_hi - s;
+    const int *e = tb_sp + 6 * seed_px[k];
+    float gs = 0.0f;
+    if (a.ybar) {
+      for (int t = e[4]; t < e[5]; ++t) {
+        const int32_t *tp = a.taps + t * TAP_W;
+        const int64_t o = ((int64_t)b * a.T + (m - 1)) * a.ntap + tp[3];
+        const float gy = a.ybar[o];
+        gs += tp[2] ? 2.0f * a.yraw[o] * gy : gy;
+      }
+    }
+    seeds[i] = gs;
+  }
+  int sr0 = -1, sr1 = -1, sd0 = -1, sd1 = -1, sl0 = -1, sl1 = -1;
+  {
+    int nseed = 0;
+    for (int kk = 0; kk < nsp; ++kk) {
+      const int *e = tb_sp + 6 * kk;
+      const bool has_tap = e[5] > e[4];
+      const int seed_slot = has_tap ? nseed : -1;
+      if (has_tap) ++nseed;
+      const int rr = e[0] - g.R0 - row0;
+      if (!g.active || e[1] - g.C0 != g.c || rr < 0 || rr >= R || row0 + rr >= g.RH) continue;
+      const int src_slot = e[2] > 0 ? e[3] : -1;
+      if (sr0 < 0) { sr0 = rr; sd0 = seed_slot; sl0 = src_slot; }
+      else { sr1 = rr; sd1 = seed_slot; sl1 = src_slot; }
+    }
+  }
+  uint32_t omask = 0;
+  if (g.active && col >= g.oc0 && col < g.oc1) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int gr = g.R0 + row0 + r;
+      if (row0 + r < g.RH && gr >= g.or0 && gr < g.or1) omask |= 1u << r;
+    }
+  }
+  __syncthreads();
+
+  const float s2 = a.s2, dtm1 = a.dtm1, dtm2 = a.dtm2;
+  const float uth = a.sat_uth, b0 = a.sat_b0, cnl = a.c_nl;
+  const int nlf = a.nlf;
+  auto step = [&](const float *P, float *Q, float(&Lx)[R], float(&Ly)[R], int s) {
+    // Lx = lambda_{m+1}, Ly = lambda_{m+2} on entry, lambda_m on exit
+    const int m = a.m_hi - s;
+    int lo, hi;
+    cone(g, s, a.nsteps, lo, hi);
+    const bool need = g.active && row0 < hi && row0 + R > lo;
+    if (need) {
+      // lap(u_{m-1}) from the record and u_{m-2} from the record, or from the
+      // checkpoint when m-2 precedes the segment (rows outside the region
+      // read nothing: vmask)
+      const __amdgpu_buffer_rsrc_t rl =
+          cs_rsrc(a.seg_lap + ((int64_t)b * a.seg_K + (m - 1 - a.seg_n0)) * F, F * 4);
+      const __amdgpu_buffer_rsrc_t ru =
+          (m - 2) < a.seg_n0
+              ? cs_rsrc(cy.uprev0 + boff, F * 4)
+              : cs_rsrc(a.seg_u + ((int64_t)b * a.seg_K + (m - 2 - a.seg_n0)) * F, F * 4);
+      float UL[R], LL[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const uint32_t o = ((vmask >> r) & 1u) ? voff + (uint32_t)(r * Ny * 4) : 0x80000000u;
+        LL[r] = cs_bload(rl, o);
+        UL[r] = cs_bload(ru, o);
+      }
+      const float gs0 = sd0 >= 0 ? seeds[sd0 * a.nsteps + s] : 0.0f;
+      const float gs1 = sd1 >= 0 ? seeds[sd1 * a.nsteps + s] : 0.0f;
+      const int lfb = g.base - g.CS, rtb = g.base + g.CS;
+      float upv = P[g.base - 1];
+      float ct = Wm[0] * Lx[0];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float dn = (r < R - 1) ? Wm[r + 1] * Lx[r + 1] : P[g.base + R];
+        const float lf = P[lfb + r], rt = P[rtb + r];
+        float lapw = upv + lf;
+        lapw = __builtin_fmaf(-4.0f, ct, lapw);
+        lapw = lapw + rt;
+        lapw = lapw + dn;
+        float lam = (Pm[r] * Lx[r] + lapw) - A2a[r] * Ly[r];
+        if (r == sr0) lam += gs0;
+        if (r == sr1) lam += gs1;
+        upv = ct;
+        ct = dn;
+        // coefficients of step m-1 at this cell (u = u_{m-1}, up = u_{m-2})
+        const float4 mt = mat[g.base + r];
+        const float u = U1[r], up = UL[r], lapu = LL[r], rho = mt.z;
+        float bb = mt.x, cc = mt.y, sd = 0.0f, c2raw = 0.0f, dbdu = 0.0f, dcdu = 0.0f;
+        if (nlf & 1) {
+          const float q = u / uth, d = 1.0f + q * q;
+          sd = b0 / d;
+          bb += rho * sd;
+          dbdu = rho * b0 * (-2.0f * q / uth) / (d * d);
+        }
+        if (nlf & 2) {
+          c2raw = cnl * (u * u);
+          cc += rho * c2raw;
+          dcdu = 2.0f * rho * cnl * u;
+        }
+        bool clamped = false;
+        if (nlf & 4) {
+          if (cc < a.cmin) { cc = a.cmin; clamped = true; }
+          if (cc > a.cmax) { cc = a.cmax; clamped = true; }
+        }
+        const float bd = bb * dtm1;
+        const float A = 1.0f / (dtm2 + bd);
+        const float Bm = dtm2 - bd;
+        const float v = A * ((s2 * u - Bm * up) + (cc * cc) * lapu);
+        const float dvdb = A * dtm1 * (up - v);
+        const float dvdc = clamped ? 0.0f : 2.0f * A * cc * lapu;
+        Gc[r] = __builtin_fmaf(dvdc, lam, Gc[r]);
+        if (GB) Gb[r] = __builtin_fmaf(dvdb, lam, Gb[r]);
+        Gr[r] = __builtin_fmaf(dvdb * sd + dvdc * c2raw, lam, Gr[r]);
+        const float wn = A * (cc * cc);
+        Wm[r] = wn;
+        Pm[r] = s2 * A + dvdb * dbdu + dvdc * dcdu;
+        A2a[r] = A2b[r];
+        A2b[r] = A * Bm;
+        U1[r] = up;
+        Ly[r] = lam;
+        Q[g.base + r] = wn * lam;
+      }
+      if (sr0 >= 0) {
+        // lambda_m at source pixels of the output rows -> dJ/dx[b, m-1]
+        float *gsrc_row = a.gsrc + ((int64_t)b * a.T + (m - 1)) * a.nsrcslot;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (r == sr0 && sl0 >= 0 && ((omask >> r) & 1u)) gsrc_row[sl0] = Ly[r];
+          if (r == sr1 && sl1 >= 0 && ((omask >> r) & 1u)) gsrc_row[sl1] = Ly[r];
+        }
+      }
+    }
+    __syncthreads();
+  };
+
+  int s = 0;
+  for (; s + 1 < a.nsteps; s += 2) {
+    step(buf0, buf1, Lc, Ln, s);      // lambda_m -> Ln
+    step(buf1, buf0, Ln, Lc, s + 1);  // lambda_{m-1} -> Lc
+  }
+  if (s < a.nsteps) step(buf0, buf1, Lc, Ln, s);
+  const bool odd = (a.nsteps & 1) != 0;
+  if (g.active) {
+    const int64_t q0 = boff + (int64_t)(g.R0 + row0) * Ny + col;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if ((omask >> r) & 1u) {
+        const int64_t q = q0 + (int64_t)r * Ny;
+        a.out1[q] = odd ? Ln[r] : Lc[r];
+        a.out2[q] = odd ? Lc[r] : Ln[r];
+        cy.w_out[q] = Wm[r];
+        cy.p_out[q] = Pm[r];
+        cy.a2a_out[q] = A2a[r];
+        cy.a2b_out[q] = A2b[r];
+        a.gacc[q] += Gc[r];
+        a.gacc_rho[q] += Gr[r];
+        if (GB) a.gacc_b[q] += Gb[r];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 
 static int cs_max_dyn_lds() {
@@ -729,6 +979,39 @@ hipError_t launch_adj_cs(int nt, int R, const AdjArgs &a, int RHmax, int RWmax, 
   WT_CS_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
+}
+
+// Nonlinear adjoint: 4 rows per thread at 1024 threads (the carried
+// coefficients fill the register file; 6 rows spill).
+bool cs_nl_variant_exists(int nt, int r) { return nt == 1024 && r == 4; }
+
+size_t cs_nl_shmem_bytes(int R, int RH, int RW) {
+  int nseg = (RH + R - 1) / R;
+  int CS = (nseg * R + 2) | 1;
+  return cs_shmem_bytes(R, RH, RW) + 16 + (size_t)(RW + 2) * CS * 16;
+}
+
+hipError_t launch_adj_cs_nl(int nt, int R, bool gb, const AdjArgs &a, const NlCarry &cy,
+                            int RHmax, int RWmax, hipStream_t s) {
+  int nblocks = a.B * a.ntiles;
+  size_t shmem = cs_nl_shmem_bytes(R, RHmax, RWmax);
+  int nthreads = cs_threads(R, RHmax, RWmax);
+  if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt || !cs_nl_variant_exists(nt, R))
+    return hipErrorInvalidConfiguration;
+#define NLK(RR, G)                                                                     \
+  do {                                                                                 \
+    auto kfn = k_adj_cs_nl<1024, RR, G>;                                               \
+    static bool done = false;                                                          \
+    if (!done) {                                                                       \
+      hipError_t e = cs_allow_lds((const void *)kfn);                                  \
+      if (e != hipSuccess) return e;                                                   \
+      done = true;                                                                     \
+    }                                                                                  \
+    hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, a, cy);           \
+  } while (0)
+  if (gb) NLK(4, true); else NLK(4, false);
+#undef NLK
+  return hipGetLastError();
 }
 
 // Batched launches (linear cell): one launch over the work items of every

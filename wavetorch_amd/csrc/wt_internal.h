@@ -112,6 +112,15 @@ struct BatchAdj {
 };
 static_assert(sizeof(BatchAdj) < 4000, "batched launch arguments exceed the kernarg limit");
 
+// Carried state of the temporally blocked nonlinear adjoint between launches
+// (B,Nx,Ny each): w_m = A c^2, P_m, a2_{m+1} and a2_m at the first step m of
+// the next launch, and u_{n0-1} of the current segment.
+struct NlCarry {
+  const float *w_in, *p_in, *a2a_in, *a2b_in;
+  float *w_out, *p_out, *a2a_out, *a2b_out;
+  const float *uprev0;
+};
+
 // Extra buffers of the nonlinear adjoint (one launch per step): products of
 // lambda_m with step m-1 factors handed to the next iterations, and where
 // u_{n0-1} of the current segment lives.

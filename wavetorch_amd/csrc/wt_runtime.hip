@@ -79,6 +79,10 @@ int cs_threads(int R, int RH, int RW);
 hipError_t launch_fwd_cs(int nt, int R, int nl, bool rec, const FwdArgs &a, int RHmax,
                          int RWmax, hipStream_t s);
 hipError_t launch_adj_cs(int nt, int R, const AdjArgs &a, int RHmax, int RWmax, hipStream_t s);
+bool cs_nl_variant_exists(int nt, int r);
+size_t cs_nl_shmem_bytes(int R, int RH, int RW);
+hipError_t launch_adj_cs_nl(int nt, int R, bool gb, const AdjArgs &a, const NlCarry &cy,
+                            int RHmax, int RWmax, hipStream_t s);
 hipError_t launch_mul_scalar(const float *in, float v, int64_t n, float *out, hipStream_t s);
 hipError_t launch_clamp(const float *in, float lo, float hi, int64_t n, float *out,
                         hipStream_t s);
@@ -344,7 +348,9 @@ static int plan_layout(Plan &p) {
   p.saved_bytes = p.saved_coef + p.saved_ck + p.saved_yraw;
   // temp
   p.t_state = align_up(8 * BF);  // 4 state pairs (ping-pong + segment in/out)
-  p.t_lam = align_up(4 * BF);    // adjoint state ping-pong (plain uses 3)
+  // adjoint state ping-pong (plain uses 3); the nonlinear blocked adjoint
+  // adds two sets of four carried coefficient fields
+  p.t_lam = align_up((nlc ? 12 : 4) * BF);
   p.t_wl = align_up(2 * BF);     // plain adjoint w*lambda
   p.t_gacc = align_up((nlc ? 3 : 1) * BF);
   size_t segcells = (size_t)p.B * p.K * p.Nx * (p.Ny + 1);
@@ -665,12 +671,32 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
     if (atb > 256) atb = 256;
     int ant = tun.a_nt > 0 ? tun.a_nt : tun.nt;
     int acp = tun.a_cpt > 0 ? tun.a_cpt : tun.cpt;
-    build_tiles(*p, as, ac, atb);
-    // an explicit variant (adjoint's own, or the forward's) is tried as is;
-    // otherwise the fewest rows per thread that fit
     bool aok = false;
-    if (tun.a_nt > 0 || (tuning && tuning->nt > 0)) aok = tiles_ok_cs(*p, ant, acp);
-    if (!aok && tun.a_nt <= 0) aok = pick_variant_cs(*p, ant, acp);
+    if (p->nl) {
+      // nonlinear adjoint (k_adj_cs_nl): 4 rows per thread at 1024 threads,
+      // 8-step blocks unless given; tiles grow until the region fits
+      ant = 1024;
+      acp = 4;
+      if (tun.a_tblock <= 0) atb = std::min(8, p->K);
+      const bool fixed = tun.a_strips > 0;
+      for (int q = as; q <= p->Nx && !aok; ++q) {
+        if (fixed && q != as) break;
+        const int qc = fixed ? ac : std::min(q, p->Ny);
+        build_tiles(*p, q, qc, atb);
+        aok = tiles_ok_cs(*p, ant, acp) &&
+              cs_nl_shmem_bytes(acp, p->RHmax, p->RWmax) + 1024 <= LDS_LIMIT;
+        if (aok) {
+          as = q;
+          ac = qc;
+        }
+      }
+    } else {
+      build_tiles(*p, as, ac, atb);
+      // an explicit variant (adjoint's own, or the forward's) is tried as is;
+      // otherwise the fewest rows per thread that fit
+      if (tun.a_nt > 0 || (tuning && tuning->nt > 0)) aok = tiles_ok_cs(*p, ant, acp);
+      if (!aok && tun.a_nt <= 0) aok = pick_variant_cs(*p, ant, acp);
+    }
     if (!aok) {
       set_error("wt_plan_create: adjoint tuning (strips=%d tcols=%d tblock=%d nt=%d cpt=%d) "
                 "does not fit", as, ac, atb, ant, acp);
@@ -948,9 +974,13 @@ int adj_nonlinear(const Plan &p, const float *b, const float *c, const float *rh
   fa.rec_lap = w.seg;
   fa.rec_u = w.seg + segcells;
   fa.rec_K = K;
-  const bool cs = p.tiled && p.tun.kind == 0;
+  // the record's layout follows the recompute kernels (column-segment rows of
+  // Ny, else Ny + 1); column-segment plans also run the blocked adjoint,
+  // except with dense field seeds, which the per-step kernel reads
+  const bool cs_rec = p.tiled && p.tun.kind == 0;
+  const bool cs = cs_rec && fbar == nullptr;
   NlAdjBufs nb{};
-  nb.rec_ld = cs ? p.Ny : p.Ny + 1;
+  nb.rec_ld = cs_rec ? p.Ny : p.Ny + 1;
   nb.rec_slice = (int64_t)p.Nx * nb.rec_ld;
   AdjArgs aa{};
   aa.B = p.B; aa.Nx = p.Nx; aa.Ny = p.Ny; aa.T = T;
@@ -964,6 +994,42 @@ int adj_nonlinear(const Plan &p, const float *b, const float *c, const float *rh
   aa.seg_lap = fa.rec_lap; aa.seg_u = fa.rec_u; aa.seg_K = K;
   aa.gacc = w.gacc; aa.gacc_b = w.gacc + BF; aa.gacc_rho = w.gacc + 2 * BF;
   aa.gsrc = w.gsrc;
+  if (cs) {
+    // temporally blocked (k_adj_cs_nl): lambdas ping-pong in lam[0..3], the
+    // carried coefficients (w, P, a2 of two steps) in two sets of four
+    WT_HIP(launch_fill(w.lam + 3 * BF, 9 * BF, 0.0f, s));  // (the first 3 are filled above)
+    float *l[4], *cA[4], *cB[4];
+    for (int i = 0; i < 4; ++i) {
+      l[i] = w.lam + i * BF;
+      cA[i] = w.lam + (4 + i) * BF;
+      cB[i] = w.lam + (8 + i) * BF;
+    }
+    aa.ntiles = p.ntiles_a;
+    aa.tiles = p.d_tiles_a;
+    for (int sg = p.nseg - 1; sg >= 0; --sg) {
+      const int n0 = sg * K, n1 = std::min(T, n0 + K);
+      const float *ck = w.ck + (size_t)sg * 2 * BF;
+      fa.rec_n0 = n0;
+      float *tmp[4] = {st[0], st[1], st[2], st[3]};
+      int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+      if (rc) return rc;
+      aa.seg_n0 = n0;
+      for (int m = n1; m > n0;) {
+        const int k = std::min(p.tb_a, m - n0);
+        aa.in1 = l[0]; aa.in2 = l[1]; aa.out1 = l[2]; aa.out2 = l[3];
+        aa.m_hi = m;
+        aa.nsteps = k;
+        NlCarry cy{cA[0], cA[1], cA[2], cA[3], cB[0], cB[1], cB[2], cB[3], ck + BF};
+        ProfScope ps(2, s, BF * k);
+        WT_HIP(launch_adj_cs_nl(p.nt_a, p.cpt_a, grad_b != nullptr, aa, cy, p.RHmax_a,
+                                p.RWmax_a, s));
+        std::swap(l[0], l[2]);
+        std::swap(l[1], l[3]);
+        for (int i = 0; i < 4; ++i) std::swap(cA[i], cB[i]);
+        m -= k;
+      }
+    }
+  } else {
   float *wl1 = w.wl, *wl2 = w.wl + BF;
   nb.PL = w.lam;
   for (int sg = p.nseg - 1; sg >= 0; --sg) {
@@ -983,6 +1049,7 @@ int adj_nonlinear(const Plan &p, const float *b, const float *c, const float *rh
       WT_HIP(launch_adj_nl_plain(p.unit_h, p.nl, aa, p.d_spmap, nb, fbar, m, s));
       std::swap(wl1, wl2);
     }
+  }
   }
   const int64_t F = (int64_t)p.F;
   if (grad_c) WT_HIP(launch_batch_sum(w.gacc, p.B, F, grad_c, s));
@@ -1166,7 +1233,7 @@ struct BatchPlan {
   std::vector<int32_t *> d_Tlim;  // per group, per instance step count (device)
   std::vector<int> Tg;            // per group longest instance
   int T = 0, K = 64, nseg = 1;
-  int tb = 16, nt = 1024, R = 10, tb_a = 12, nt_a = 1024, R_a = 8;
+  int tb = 24, nt = 1024, R = 10, tb_a = 16, nt_a = 1024, R_a = 8;
   int RHmax = 0, RWmax = 0, RHmax_a = 0, RWmax_a = 0;
   std::vector<size_t> soff, toff;
   size_t saved_bytes = 0, temp_bytes = 0;
