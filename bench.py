@@ -42,15 +42,20 @@ def traffic_file(workload):
     return os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
 
 
-def class_name(cls, tun):
+def class_name(cls, tun, nonlinear=False):
     """Kernel (as rocprofv3 names it) of a kernel class under plan tuning `tun`."""
     if tun["plain"]:
-        return ["wt::k_fwd_plain", "wt::k_fwd_plain (record)", "wt::k_adj_plain"][cls]
+        return ["wt::k_loop_fwd_plain", "wt::k_loop_fwd_plain (record)",
+                "wt::k_loop_adj_nl_plain" if nonlinear else "wt::k_loop_adj_plain"][cls]
     if tun["kind"] == 1:
-        return ["wt::k_fwd_tiled", "wt::k_fwd_tiled (record)", "wt::k_adj_tiled"][cls]
+        return ["wt::k_fwd_tiled", "wt::k_fwd_tiled (record)",
+                "wt::k_adj_nl_plain" if nonlinear else "wt::k_adj_tiled"][cls]
     if cls == 2:
+        if nonlinear:
+            return f"wt::k_adj_cs_nl<{tun['a_nt']}, {tun['a_cpt']}, false>"
         return f"wt::k_adj_cs<{tun['a_nt']}, {tun['a_cpt']}>"
-    return f"wt::k_fwd_cs<{tun['nt']}, {tun['cpt']}, 0, {'true' if cls == 1 else 'false'}>"
+    return f"wt::k_fwd_cs<{tun['nt']}, {tun['cpt']}, {8 if nonlinear else 0}, " \
+        f"{'true' if cls == 1 else 'false'}>"
 
 
 def log(*a):
@@ -481,10 +486,11 @@ def main():
     avg_s = ms / max(nl, 1) / 1e3
     per_launch = work / max(nl, 1)
     achieved = BYTES[cls] * per_launch / avg_s
+    nonlin = plan.cfg.nonlinear
     for c, (m_, n_, w_) in prof.items():
-        log(f"[bench] class {c} {class_name(c, plan.tuning)}: {n_} launches, {m_:.2f} ms total, "
-            f"{(m_ / max(n_, 1)) * 1e3:.1f} us avg")
-    kname = class_name(cls, plan.tuning)
+        log(f"[bench] class {c} {class_name(c, plan.tuning, nonlin)}: {n_} launches, "
+            f"{m_:.2f} ms total, {(m_ / max(n_, 1)) * 1e3:.1f} us avg")
+    kname = class_name(cls, plan.tuning, nonlin)
     out = {
         "metric": wl["metric"],
         "value": round(value, 3),
@@ -506,7 +512,7 @@ def main():
                      "kernel": kname, "bytes_per_cell_step": BYTES[cls],
                      "cell_steps_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
                      "launches": nl, "events_ms_per_step": round(prof_step_s * 1e3, 3),
-                     "kernel_ms_per_step": {class_name(c, plan.tuning):
+                     "kernel_ms_per_step": {class_name(c, plan.tuning, nonlin):
                                             round(prof[c][0] / args.steps, 3)
                                             for c in prof if prof[c][1]}},
         # the whole step at its algorithmic bytes per cell-step (28 B = 12 forward

@@ -70,10 +70,10 @@ def cached(workload: str):
 # Options of each tuning field the search moves through (column-segment
 # family; rows per thread `cpt` must be a compiled variant).
 AXES = {
-    "tiles": (1, 2, 3, 4),           # g x g tiles per sample (forward)
-    "tblock": (8, 12, 16, 20, 24, 32),
-    "cpt": (6, 8, 10, 12),
-    "a_tiles": (1, 2, 3, 4),         # g x g tiles per sample (adjoint)
+    "tiles": (1, 2, 3, 4, 5, 6),     # g x g tiles per sample (forward)
+    "tblock": (4, 6, 8, 12, 16, 20, 24, 32),
+    "cpt": (4, 6, 8, 10, 12),
+    "a_tiles": (1, 2, 3, 4, 5, 6),   # g x g tiles per sample (adjoint)
     "a_tblock": (6, 8, 10, 12, 13, 16),
     "a_cpt": (4, 6, 8),
 }
@@ -137,18 +137,30 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
         return None
     state = _from_tuning(base)
     tried = {tuple(sorted(state.items()))}
+    def consider(cand):
+        nonlocal state, best_t
+        key_ = tuple(sorted(cand.items()))
+        if key_ in tried or time.perf_counter() - t_start > budget_s:
+            return False
+        tried.add(key_)
+        _, t = measure(_to_tuning(cand))
+        if t < best_t * 0.995:
+            state, best_t = cand, t
+            return True
+        return False
+
     for _ in range(rounds):
         improved = False
         for field, options in AXES.items():
             for v in options:
-                cand = dict(state, **{field: v})
-                key_ = tuple(sorted(cand.items()))
-                if key_ in tried or time.perf_counter() - t_start > budget_s:
-                    continue
-                tried.add(key_)
-                _, t = measure(_to_tuning(cand))
-                if t < best_t * 0.995:
-                    state, best_t, improved = cand, t, True
+                improved |= consider(dict(state, **{field: v}))
+        # block depth and rows per thread are coupled (deeper blocks need more
+        # rows per thread to fit): also move them together
+        for f1, f2 in (("tblock", "cpt"), ("a_tblock", "a_cpt")):
+            base_state = dict(state)
+            for v1 in AXES[f1]:
+                for v2 in AXES[f2]:
+                    improved |= consider(dict(base_state, **{f1: v1, f2: v2}))
         if not improved:
             break
     best = _to_tuning(state)

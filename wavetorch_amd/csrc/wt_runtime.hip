@@ -564,17 +564,21 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
   bool tiled = !tun.plain && p->unit_h;
   if (tiled) {
     int strips = tun.strips, tb = tun.tblock, nt = tun.nt, cpt = tun.cpt;
-    if (tb <= 0) tb = 16;
+    // the nonlinear cell forms its coefficients (three divisions) per cell
+    // and step: halo work costs more there, so shallower blocks and smaller
+    // tiles (two workgroups per CU) pay
+    if (tb <= 0) tb = p->nl ? 8 : 16;
     if (tb > p->K) tb = p->K;
     if (tun.kind == 0 && tb > 256) tb = 256;  // inputs/seeds of a launch are staged in LDS
     // Heuristic for the column-segment family: g x g square tiles per
-    // sample, g the smallest giving one workgroup per CU (B*g*g >= 256)
-    // while a tile's own rows stay at least twice the halo; grown further
-    // only if no block variant holds the tile.
+    // sample, g the smallest giving one workgroup per CU (B*g*g >= 256; 512
+    // for the nonlinear cell) while a tile's own rows stay at least twice the
+    // halo; grown further only if no block variant holds the tile.
     bool square = tun.kind == 0 && strips <= 0 && tun.tcols <= 0;
     if (square) {
+      const int64_t target = p->nl ? 512 : 256;
       strips = 1;
-      while ((int64_t)B * strips * strips < 256 &&
+      while ((int64_t)B * strips * strips < target &&
              std::min(p->Nx, p->Ny) / (strips + 1) >= 2 * tb)
         ++strips;
     } else if (strips <= 0) {
@@ -605,7 +609,8 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
       }
       tiled = false;
     } else {
-      if (tun.kind == 0 && !(tuning && tuning->tblock > 0) && !(tuning && tuning->nt > 0)) {
+      if (tun.kind == 0 && p->nl == 0 && !(tuning && tuning->tblock > 0) &&
+          !(tuning && tuning->nt > 0)) {
         // deepest forward block (fewest launches per checkpoint segment)
         // whose regions still fit 1024 threads at <= 10 rows per thread,
         // while a tile's own rows and columns stay at least twice its halo
