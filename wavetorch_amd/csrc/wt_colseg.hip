@@ -86,12 +86,13 @@ __device__ __forceinline__ void cs_bstore(__amdgpu_buffer_rsrc_t rs, uint32_t of
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, (int)off, 0, 0);
 }
 // The segment record (lap u, u) is written once by the recompute and read
-// once by the adjoint.  Its reads carry the non-temporal policy (cache
-// policy operand 2 = nt), which measured 6% faster adjoint launches on
-// MI355X; its writes keep the default policy (nt writes doubled the
-// recompute's time: the record rows are narrower than a cache line).
+// once by the adjoint.  Cache policies of those accesses (the buffer
+// instructions' cache-policy operand: 2 = nt, 16 = sc1).  Measured on MI355X
+// for the training step: nt writes double the recompute's time (the record
+// rows are narrower than a cache line); nt reads with default writes slow
+// the adjoint by 8%; the default policy on both is fastest overall.
 #ifndef WT_RECORD_LOAD_CPOL
-#define WT_RECORD_LOAD_CPOL 2
+#define WT_RECORD_LOAD_CPOL 0
 #endif
 #ifndef WT_RECORD_STORE_CPOL
 #define WT_RECORD_STORE_CPOL 0
