@@ -55,13 +55,13 @@ def c2_cfg(N=150):
 
 def test_resolve_training_shape_heuristic():
     """Batch 64 on 150x150: 2x2 tiles per sample (256 workgroups), the
-    deepest blocks that fit 1024 threads -- forward 24 steps at 10 rows per
-    thread, adjoint 13 steps at 8 (what the autotuner also finds best)."""
+    deepest blocks that fit 1024 threads -- forward 32 steps at 12 rows per
+    thread, adjoint 24 steps at 10 (what the autotuner also finds best)."""
     t, saved, temp = resolve(c2_cfg(), 64, 3000, 1.0)
     assert (t["plain"], t["kind"], t["strips"], t["tcols"]) == (0, 0, 2, 2)
-    assert (t["tblock"], t["nt"], t["cpt"]) == (24, 1024, 10)
+    assert (t["tblock"], t["nt"], t["cpt"]) == (32, 1024, 12)
     assert (t["a_strips"], t["a_tcols"], t["a_tblock"], t["a_nt"], t["a_cpt"]) == \
-        (2, 2, 13, 1024, 8)
+        (2, 2, 24, 1024, 10)
     assert t["ck_every"] == 64
     F = 150 * 150
     # checkpoints (u_n, u_{n-1}) for ceil(T/K) segments dominate the saved bytes

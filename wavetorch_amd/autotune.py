@@ -74,8 +74,8 @@ AXES = {
     "tblock": (4, 6, 8, 12, 16, 20, 24, 32),
     "cpt": (4, 6, 8, 10, 12),
     "a_tiles": (1, 2, 3, 4, 5, 6),   # g x g tiles per sample (adjoint)
-    "a_tblock": (6, 8, 10, 12, 13, 16),
-    "a_cpt": (4, 6, 8),
+    "a_tblock": (6, 8, 10, 12, 13, 16, 20, 24),
+    "a_cpt": (4, 6, 8, 10),
 }
 
 
@@ -176,8 +176,8 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
 
 BATCH_CANDIDATES = [
     {"tblock": tb, "nt": 1024, "cpt": r, "a_tblock": atb, "a_nt": 1024, "a_cpt": ar}
-    for tb, r in ((16, 10), (16, 12), (24, 10), (24, 12), (32, 12))
-    for atb, ar in ((12, 8), (13, 8), (16, 8))
+    for tb, r in ((16, 10), (24, 10), (24, 12), (32, 12))
+    for atb, ar in ((13, 8), (16, 8), (16, 10), (20, 10), (24, 10))
 ]
 
 

@@ -381,11 +381,7 @@ __global__ __launch_bounds__(NT) void k_fwd_cs_batch(BatchFwd bl) {
 constexpr int CS_MAX_SEEDS = 16;
 
 // One reverse step of this thread's segment.  SP: the wave holds
-// probe/source pixels (sr0, sr1).  SGc holds lap(u_{m-1}) of this step
-// (prefetched one step earlier); the loads of the next step's values go
-// into SGn first, so their latency is covered by a whole step.  The record
-// is read through a range-checked buffer based at the tile's first output
-// row (zero outside the output rows).  a2 = A*Bm is formed as a1 - 1 (with
+// probe/source pixels (sr0, sr1).  a2 = A*Bm is formed as a1 - 1 (with
 // a1 = A*s2 = 2/(1+b*dt), a2 = (1-b*dt)/(1+b*dt)).  The thread's own w*lambda
 // values (up/centre/down neighbours) are re-formed as KW*Lc -- the same fp32
 // product the previous step stored to LDS -- instead of being kept in R more
@@ -393,15 +389,9 @@ constexpr int CS_MAX_SEEDS = 16;
 template <int R, bool SP>
 __device__ __forceinline__ void cs_adj_rows(
     const float *__restrict__ P, float *__restrict__ Q, float (&Lc)[R], float (&Ln)[R],
-    const float (&KA1)[R], const float (&KW)[R], float (&G)[R],
-    const float (&SGc)[R], float (&SGn)[R], int base, int CS, __amdgpu_buffer_rsrc_t rsn,
-    bool prefetch, uint32_t voff, int Ny, uint32_t omask, int sr0, int sr1, float gs0,
-    float gs1, int sl0, int sl1, float *__restrict__ gsrc_row) {
+    const float (&KA1)[R], const float (&KW)[R], int base, int CS, uint32_t omask, int sr0,
+    int sr1, float gs0, float gs1, int sl0, int sl1, float *__restrict__ gsrc_row) {
   const int lfb = base - CS, rtb = base + CS;
-  if (prefetch) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) SGn[r] = cs_rec_load(rsn, voff + (uint32_t)(r * Ny * 4));
-  }
   float upv = P[base - 1];
   float ct = KW[0] * Lc[0];
 #pragma unroll
@@ -422,7 +412,6 @@ __device__ __forceinline__ void cs_adj_rows(
     ct = dn;
     Ln[r] = lam;
     Q[base + r] = KW[r] * lam;
-    G[r] = __builtin_fmaf(SGc[r], lam, G[r]);
   }
   if (SP) {
     // lambda_m at source pixels of the output rows -> dJ/dx[b, m-1]
@@ -556,32 +545,59 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
         a.seg_lap + ((int64_t)b * a.seg_K + (m - 1 - a.seg_n0)) * F + (int64_t)g.or0 * Ny,
         out_bytes);
   };
-  // lap(u_{m-1}) of the first step of this launch
+  // lap(u_{m-1}) of the first two steps of this launch (SG0: even steps,
+  // SG1: odd steps)
   {
     const __amdgpu_buffer_rsrc_t rs0 = seg_rsrc(a.m_hi);
 #pragma unroll
     for (int r = 0; r < R; ++r) SG0[r] = cs_rec_load(rs0, voff + (uint32_t)(r * Ny * 4));
+    // issued even for a one-step launch (out of range, reads nothing) so
+    // that the loop's wait counts need not cover a path without them
+    const bool two = a.nsteps > 1;
+    const __amdgpu_buffer_rsrc_t rs1 = seg_rsrc(two ? a.m_hi - 1 : a.m_hi);
+    const uint32_t voff1 = two ? voff : 0x80000000u;
+#pragma unroll
+    for (int r = 0; r < R; ++r) SG1[r] = cs_rec_load(rs1, voff1 + (uint32_t)(r * Ny * 4));
   }
 
-  auto step = [&](const float *P, float *Q, float(&Lc)[R], float(&Ln)[R], const float(&SGc)[R],
-                  float(&SGn)[R], int s) {
+  auto step = [&](const float *P, float *Q, float(&Lc)[R], float(&Ln)[R], float(&SG)[R],
+                  int s) {
     const int m = a.m_hi - s;
     int lo, hi;
     cone(g, s, a.nsteps, lo, hi);
     const bool need = g.active && row0 < hi && row0 + R > lo;
-    const bool prefetch = s + 1 < a.nsteps;
-    const __amdgpu_buffer_rsrc_t rsn = seg_rsrc(prefetch ? m - 1 : m);
+    // record index m-3-seg_n0 >= 0: m-2 >= m_hi-nsteps+1 >= seg_n0+1
+    const bool prefetch = s + 2 < a.nsteps;
+    const __amdgpu_buffer_rsrc_t rsn = seg_rsrc(prefetch ? m - 2 : m);
+    const uint32_t pvoff = prefetch ? voff : 0x80000000u;
     float *gsrc_row = a.gsrc + ((int64_t)b * a.T + (m - 1)) * a.nsrcslot;
     if (need) {
       if (wave_sp) {
         const float gs0 = sd0 >= 0 ? seeds[sd0 * a.nsteps + s] : 0.0f;
         const float gs1 = sd1 >= 0 ? seeds[sd1 * a.nsteps + s] : 0.0f;
-        cs_adj_rows<R, true>(P, Q, Lc, Ln, KA1, KW, G, SGc, SGn, g.base, g.CS, rsn, prefetch,
-                             voff, Ny, omask, sr0, sr1, gs0, gs1, sl0, sl1, gsrc_row);
+        cs_adj_rows<R, true>(P, Q, Lc, Ln, KA1, KW, g.base, g.CS, omask, sr0, sr1, gs0, gs1,
+                             sl0, sl1, gsrc_row);
       } else {
-        cs_adj_rows<R, false>(P, Q, Lc, Ln, KA1, KW, G, SGc, SGn, g.base, g.CS, rsn,
-                              prefetch, voff, Ny, omask, -1, -1, 0.f, 0.f, -1, -1, gsrc_row);
+        cs_adj_rows<R, false>(P, Q, Lc, Ln, KA1, KW, g.base, g.CS, omask, -1, -1, 0.f, 0.f,
+                              -1, -1, gsrc_row);
       }
+    }
+    // gradient sum, then refill SG with lap(u) of step m-2 (the record is
+    // read through a range-checked buffer based at the tile's first output
+    // row, zero outside the output rows): one issue point
+    // outside the branches, so the loads stay in flight across two steps.
+    // Threads whose segment left the cone (or own no cells) accumulate
+    // stale values into rows that are not output rows and never written.
+    // The loads are always issued (pvoff is out of range when there is no
+    // step m-2 in this launch) so that the wait counts stay exact.
+    // The empty asm keeps each sum in this step: deferred into the next
+    // step, the old and the refilled SG would both be live and the loop
+    // would copy the fresh loads (waiting for them) at its back edge.
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      G[r] = __builtin_fmaf(SG[r], Ln[r], G[r]);
+      asm volatile("" : "+v"(G[r]));
+      SG[r] = cs_rec_load(rsn, pvoff + (uint32_t)(r * Ny * 4));
     }
     __syncthreads();
   };
@@ -589,10 +605,10 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
   WT_TS(1);
   int s = 0;
   for (; s + 1 < a.nsteps; s += 2) {
-    step(buf0, buf1, L1, L2, SG0, SG1, s);   // lambda_m -> L2
-    step(buf1, buf0, L2, L1, SG1, SG0, s + 1);
+    step(buf0, buf1, L1, L2, SG0, s);   // lambda_m -> L2
+    step(buf1, buf0, L2, L1, SG1, s + 1);
   }
-  if (s < a.nsteps) step(buf0, buf1, L1, L2, SG0, SG1, s);
+  if (s < a.nsteps) step(buf0, buf1, L1, L2, SG0, s);
   WT_TS(2);
   const bool odd = (a.nsteps & 1) != 0;
   if (g.active) {

@@ -612,7 +612,7 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
       if (tun.kind == 0 && p->nl == 0 && !(tuning && tuning->tblock > 0) &&
           !(tuning && tuning->nt > 0)) {
         // deepest forward block (fewest launches per checkpoint segment)
-        // whose regions still fit 1024 threads at <= 10 rows per thread,
+        // whose regions still fit 1024 threads at <= 12 rows per thread,
         // while a tile's own rows and columns stay at least twice its halo
         const int own = std::min(p->Nx / strips, p->Ny / tcols);
         // a single tile per sample has no halo: one launch per segment
@@ -621,7 +621,7 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
           if (d <= tb || d > p->K || (d != whole && own < 2 * d)) continue;
           build_tiles(*p, strips, tcols, d);
           int r = 0;
-          if (fit_cs(*p, 10, r)) {
+          if (fit_cs(*p, 12, r)) {
             tb = d;
             cpt = r;
             nt = 1024;
@@ -653,7 +653,9 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
     int as = tun.a_strips > 0 ? tun.a_strips : tun.strips;
     int ac = tun.a_tcols > 0 ? tun.a_tcols : (tun.tcols > 0 ? tun.tcols : 1);
     // the adjoint holds three fields per row in registers: by default the
-    // deepest block whose tiles fit 1024 threads at <= 8 rows per thread
+    // deepest block whose tiles fit 1024 threads at <= 10 rows per thread
+    // (deeper blocks, fewer launches: each launch pays a set-up and a
+    // write-back of its region)
     int atb = tun.a_tblock > 0 ? tun.a_tblock
                                : (tuning && tuning->tblock > 0 ? tun.tblock
                                                                : std::min(tun.tblock, 12));
@@ -662,11 +664,11 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
       const int ac0 = tun.a_tcols > 0 ? tun.a_tcols : (tun.tcols > 0 ? tun.tcols : 1);
       const int own = std::min(p->Nx / as0, p->Ny / ac0);
       const int whole = (as0 == 1 && ac0 == 1) ? std::min(p->K, 256) : 0;
-      for (int d : {whole, 16, 15, 14, 13, 12, 10, 8, 6, 4, 2, 1}) {
+      for (int d : {whole, 24, 22, 20, 18, 16, 15, 14, 13, 12, 10, 8, 6, 4, 2, 1}) {
         if (d <= 0 || d > p->K || (d != whole && d > 1 && own < 2 * d)) continue;
         build_tiles(*p, as0, ac0, d);
         int r = 0;
-        if (fit_cs(*p, 8, r)) {
+        if (fit_cs(*p, 10, r)) {
           atb = d;
           break;
         }
@@ -1238,7 +1240,7 @@ struct BatchPlan {
   std::vector<int32_t *> d_Tlim;  // per group, per instance step count (device)
   std::vector<int> Tg;            // per group longest instance
   int T = 0, K = 64, nseg = 1;
-  int tb = 24, nt = 1024, R = 10, tb_a = 16, nt_a = 1024, R_a = 8;
+  int tb = 24, nt = 1024, R = 10, tb_a = 24, nt_a = 1024, R_a = 10;
   int RHmax = 0, RWmax = 0, RHmax_a = 0, RWmax_a = 0;
   std::vector<size_t> soff, toff;
   size_t saved_bytes = 0, temp_bytes = 0;
