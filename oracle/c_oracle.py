@@ -34,7 +34,8 @@ def build(verbose: bool = False) -> str:
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        # a library older than its source would bind the wrong signatures
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
             build()
         L = ctypes.CDLL(LIB)
         P = ctypes.c_void_p
@@ -43,9 +44,9 @@ def lib():
         d = ctypes.c_double
         L.wo_laplacian_f32.argtypes = [P, P, i, i, i, f]
         L.wo_laplacian_f64.argtypes = [P, P, i, i, i, d]
-        L.wo_forward_f32.argtypes = [i, i, i, i, P, P, f, f, P, i, P, i, P, P, P, i, P,
+        L.wo_forward_f32.argtypes = [i, i, i, i, P, P, f, f, P, i, P, i, P, P, P, i, i, P,
                                      f, f, f, f, f]
-        L.wo_forward_f64.argtypes = [i, i, i, i, P, P, d, d, P, i, P, i, P, P, P, i, P,
+        L.wo_forward_f64.argtypes = [i, i, i, i, P, P, d, d, P, i, P, i, P, P, P, i, i, P,
                                      d, d, d, d, d]
         L.wo_adjoint_f64.argtypes = [i, i, i, i, P, P, d, d, P, i, P, i, P, i, P,
                                      d, d, d, d, d, P, P, P, P, P, P]
@@ -93,11 +94,14 @@ def _nl_args(nl):
     return flags, nl["rho"], nl.get("sat_b0", 0.0), nl.get("sat_uth", 1.0), nl.get("c_nl", 0.0), cmin, cmax
 
 
-def forward(b, c, x, dt, h, src_px, taps, fields=False, nl=None, dtype=np.float32):
+def forward(b, c, x, dt, h, src_px, taps, fields=False, nl=None, dtype=np.float32,
+            field_stride=1):
     """rnn.py:21-72 forward.
 
     b, c: (Nx,Ny); x: (B,T); src_px: (n,3) int (x, y, source index);
-    taps: (m,3) int (x, y, kind).  Returns (y (B,T,m), fields (B,T,Nx,Ny) or None).
+    taps: (m,3) int (x, y, kind).  Returns (y (B,T,m), fields or None), the
+    fields being u after the steps n with (n+1) % field_stride == 0, shaped
+    (B, T // field_stride, Nx, Ny).
     """
     b = _c(b, dtype)
     c = _c(c, dtype)
@@ -107,12 +111,13 @@ def forward(b, c, x, dt, h, src_px, taps, fields=False, nl=None, dtype=np.float3
     src_px = np.ascontiguousarray(np.asarray(src_px, dtype=np.int32).reshape(-1, 3))
     taps = np.ascontiguousarray(np.asarray(taps, dtype=np.int32).reshape(-1, 3))
     y = np.zeros((B, T, len(taps)), dtype=dtype)
-    fo = np.zeros((B, T, Nx, Ny), dtype=dtype) if fields else None
+    fo = np.zeros((B, T // field_stride, Nx, Ny), dtype=dtype) if fields else None
     flags, rho, sb0, suth, cnl, cmin, cmax = _nl_args(nl)
     rho = _c(rho, dtype) if rho is not None else None
     fn = lib().wo_forward_f32 if dtype == np.float32 else lib().wo_forward_f64
     rc = fn(B, Nx, Ny, T, _p(b), _p(c), dt, h, _p(x), len(src_px), _p(src_px), len(taps),
-            _p(taps), _p(y), _p(fo), flags, _p(rho), sb0, suth, cnl, cmin, cmax)
+            _p(taps), _p(y), _p(fo), int(field_stride), flags, _p(rho), sb0, suth, cnl, cmin,
+            cmax)
     if rc != 0:
         raise MemoryError("wo_forward failed")
     return y, fo

@@ -90,16 +90,20 @@ void wo_laplacian_f64(const double *u, double *out, int B, int Nx, int Ny,
  * nl flags: bit0 = saturable damping active (satdamp_b0 > 0, cell.py:94)
  *           bit1 = Kerr-like speed active   (c_nl != 0,        cell.py:99)
  *           bit2 = clamp c to [cmin, cmax]  (extension; not in the reference)
- * fields_out: if non-NULL, receives u_{n+1} for n = 0..T-1 as (B,T,Nx,Ny)
- *             (rnn.py:66 output_fields path).
+ * fields_out: if non-NULL, receives u_{n+1} for the steps n with
+ *             (n+1) % fstride == 0 as (B, T/fstride, Nx, Ny) (rnn.py:66
+ *             output_fields path; fstride 1 = every step).
  */
 #define DEF_FWD(T, SUF)                                                        \
   int wo_forward_##SUF(int B, int Nx, int Ny, int Tn, const T *bfield,         \
                        const T *cfield, T dt, T h, const T *x, int nsrcpx,     \
                        const int32_t *src_px, int ntap, const int32_t *taps,   \
-                       T *y_out, T *fields_out, int nl, const T *rho,          \
-                       T sat_b0, T sat_uth, T c_nl, T cmin, T cmax) {          \
+                       T *y_out, T *fields_out, int fstride, int nl,           \
+                       const T *rho, T sat_b0, T sat_uth, T c_nl, T cmin,      \
+                       T cmax) {                                               \
     size_t F = (size_t)Nx * Ny;                                                \
+    if (fstride < 1) return -2;                                                \
+    const int nsnap = Tn / fstride;                                            \
     T *u1 = (T *)calloc(B * F, sizeof(T));                                     \
     T *u2 = (T *)calloc(B * F, sizeof(T));                                     \
     T *un = (T *)calloc(B * F, sizeof(T));                                     \
@@ -161,9 +165,9 @@ void wo_laplacian_f64(const double *u, double *out, int B, int Nx, int Ny,
           if (taps[3 * k + 2] == 1) v = v * v;                                 \
           if (y_out) y_out[((size_t)b * Tn + n) * ntap + k] = v;               \
         }                                                                      \
-        if (fields_out)                                                        \
-          memcpy(fields_out + ((size_t)b * Tn + n) * F, u1 + b * F,            \
-                 F * sizeof(T));                                               \
+        if (fields_out && (n + 1) % fstride == 0)                              \
+          memcpy(fields_out + ((size_t)b * nsnap + (n + 1) / fstride - 1) * F,   \
+                 u1 + b * F, F * sizeof(T));                                   \
       }                                                                        \
     }                                                                          \
     free(u1); free(u2); free(un); free(lp);                                    \

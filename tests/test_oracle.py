@@ -84,6 +84,22 @@ def test_loop_golden_bitwise(gold):
     assert np.array_equal(y, gold["loop_y"])
 
 
+def test_field_stride_selects_snapshots():
+    """field_stride k keeps the fields after steps k-1, 2k-1, ... of the
+    every-step output, in fp32 and fp64."""
+    rng = np.random.default_rng(11)
+    b = np.zeros((12, 10), np.float32)
+    c = rng.uniform(0.5, 1.0, (12, 10)).astype(np.float32)
+    x = rng.standard_normal((2, 23)).astype(np.float32)
+    for dt in (np.float32, np.float64):
+        _, f1 = co.forward(b, c, x, 0.6, 1.0, [[5, 4, 0]], [], fields=True, dtype=dt)
+        for k in (1, 5, 23):
+            _, fk = co.forward(b, c, x, 0.6, 1.0, [[5, 4, 0]], [], fields=True, dtype=dt,
+                               field_stride=k)
+            assert fk.shape == (2, 23 // k, 12, 10)
+            assert np.array_equal(fk, f1[:, k - 1::k][:, :23 // k])
+
+
 def test_loop_torch_restatement_bitwise():
     p = small_problem(T=80)
     src = [(torch.tensor(s[0]), torch.tensor(s[1])) for s in p["src"]]
