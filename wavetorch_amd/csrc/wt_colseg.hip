@@ -414,12 +414,18 @@ __device__ __forceinline__ void cs_adj_rows(
     Q[base + r] = KW[r] * lam;
   }
   if (SP) {
-    // lambda_m at source pixels of the output rows -> dJ/dx[b, m-1]
+    // lambda_m at source pixels of the output rows -> dJ/dx[b, m-1]: pick
+    // the two rows' values with selects, then at most two stores (one
+    // masked store per row would put R branches on every step of the
+    // waves that hold special pixels, which the step's barrier waits for)
+    float v0 = Ln[0], v1 = Ln[0];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (r == sr0 && sl0 >= 0 && ((omask >> r) & 1u)) gsrc_row[sl0] = Ln[r];
-      if (r == sr1 && sl1 >= 0 && ((omask >> r) & 1u)) gsrc_row[sl1] = Ln[r];
+    for (int r = 1; r < R; ++r) {
+      v0 = (r == sr0) ? Ln[r] : v0;
+      v1 = (r == sr1) ? Ln[r] : v1;
     }
+    if (sl0 >= 0 && sr0 >= 0 && ((omask >> sr0) & 1u)) gsrc_row[sl0] = v0;
+    if (sl1 >= 0 && sr1 >= 0 && ((omask >> sr1) & 1u)) gsrc_row[sl1] = v1;
   }
 }
 
