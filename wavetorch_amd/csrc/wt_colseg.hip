@@ -779,6 +779,29 @@ __global__ __launch_bounds__(NT) void k_adj_cs_nl(AdjArgs a, NlCarry cy) {
   const float s2 = a.s2, dtm1 = a.dtm1, dtm2 = a.dtm2;
   const float uth = a.sat_uth, b0 = a.sat_b0, cnl = a.c_nl;
   const int nlf = a.nlf;
+  // Record values of the step being formed: LL = lap(u_{m-1}) and UL =
+  // u_{m-2} (from the checkpoint's u_{n0-1} when m-2 precedes the segment).
+  // Each step loads those of the next step at its end, outside every branch
+  // and always issued (out of range, reading nothing, past the launch's last
+  // step or outside the region: vmask), so that they travel during the
+  // barrier and the next step's lambda pass.
+  float LL[R], UL[R];
+  auto load_record = [&](int m, bool valid) {
+    const __amdgpu_buffer_rsrc_t rl =
+        cs_rsrc(a.seg_lap + ((int64_t)b * a.seg_K + (valid ? m - 1 - a.seg_n0 : 0)) * F, F * 4);
+    const __amdgpu_buffer_rsrc_t ru =
+        (!valid || (m - 2) < a.seg_n0)
+            ? cs_rsrc(cy.uprev0 + boff, F * 4)
+            : cs_rsrc(a.seg_u + ((int64_t)b * a.seg_K + (m - 2 - a.seg_n0)) * F, F * 4);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t o =
+          (valid && ((vmask >> r) & 1u)) ? voff + (uint32_t)(r * Ny * 4) : 0x80000000u;
+      LL[r] = cs_rec_load(rl, o);
+      UL[r] = cs_rec_load(ru, o);
+    }
+  };
+  load_record(a.m_hi, true);
   auto step = [&](const float *P, float *Q, float(&Lx)[R], float(&Ly)[R], int s) {
     // Lx = lambda_{m+1}, Ly = lambda_{m+2} on entry, lambda_m on exit
     const int m = a.m_hi - s;
@@ -786,25 +809,10 @@ __global__ __launch_bounds__(NT) void k_adj_cs_nl(AdjArgs a, NlCarry cy) {
     cone(g, s, a.nsteps, lo, hi);
     const bool need = g.active && row0 < hi && row0 + R > lo;
     if (need) {
-      // lap(u_{m-1}) from the record and u_{m-2} from the record, or from the
-      // checkpoint when m-2 precedes the segment (rows outside the region
-      // read nothing: vmask)
-      const __amdgpu_buffer_rsrc_t rl =
-          cs_rsrc(a.seg_lap + ((int64_t)b * a.seg_K + (m - 1 - a.seg_n0)) * F, F * 4);
-      const __amdgpu_buffer_rsrc_t ru =
-          (m - 2) < a.seg_n0
-              ? cs_rsrc(cy.uprev0 + boff, F * 4)
-              : cs_rsrc(a.seg_u + ((int64_t)b * a.seg_K + (m - 2 - a.seg_n0)) * F, F * 4);
-      float UL[R], LL[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const uint32_t o = ((vmask >> r) & 1u) ? voff + (uint32_t)(r * Ny * 4) : 0x80000000u;
-        LL[r] = cs_rec_load(rl, o);
-        UL[r] = cs_rec_load(ru, o);
-      }
       const float gs0 = sd0 >= 0 ? seeds[sd0 * a.nsteps + s] : 0.0f;
       const float gs1 = sd1 >= 0 ? seeds[sd1 * a.nsteps + s] : 0.0f;
       const int lfb = g.base - g.CS, rtb = g.base + g.CS;
+      // 1) lambda_m of every row (needs only this step's coefficients)
       float upv = P[g.base - 1];
       float ct = Wm[0] * Lx[0];
 #pragma unroll
@@ -820,15 +828,21 @@ __global__ __launch_bounds__(NT) void k_adj_cs_nl(AdjArgs a, NlCarry cy) {
         if (r == sr1) lam += gs1;
         upv = ct;
         ct = dn;
-        // coefficients of step m-1 at this cell (u = u_{m-1}, up = u_{m-2})
+        Ly[r] = lam;
+      }
+      // 2) coefficients of step m-1 at each cell (u = u_{m-1}, up = u_{m-2}),
+      // that step's parameter gradients, w_{m-1} * lambda_m to LDS
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float lam = Ly[r];
         const float4 mt = mat[g.base + r];
         const float u = U1[r], up = UL[r], lapu = LL[r], rho = mt.z;
         float bb = mt.x, cc = mt.y, sd = 0.0f, c2raw = 0.0f, dbdu = 0.0f, dcdu = 0.0f;
         if (nlf & 1) {
-          const float q = u / uth, d = 1.0f + q * q;
-          sd = b0 / d;
+          const float q = adj_div(u, uth), d = 1.0f + q * q;
+          sd = adj_div(b0, d);
           bb += rho * sd;
-          dbdu = rho * b0 * (-2.0f * q / uth) / (d * d);
+          dbdu = adj_div(rho * b0 * adj_div(-2.0f * q, uth), d * d);
         }
         if (nlf & 2) {
           c2raw = cnl * (u * u);
@@ -841,7 +855,7 @@ __global__ __launch_bounds__(NT) void k_adj_cs_nl(AdjArgs a, NlCarry cy) {
           if (cc > a.cmax) { cc = a.cmax; clamped = true; }
         }
         const float bd = bb * dtm1;
-        const float A = 1.0f / (dtm2 + bd);
+        const float A = adj_div(1.0f, dtm2 + bd);
         const float Bm = dtm2 - bd;
         const float v = A * ((s2 * u - Bm * up) + (cc * cc) * lapu);
         const float dvdb = A * dtm1 * (up - v);
@@ -855,19 +869,23 @@ __global__ __launch_bounds__(NT) void k_adj_cs_nl(AdjArgs a, NlCarry cy) {
         A2a[r] = A2b[r];
         A2b[r] = A * Bm;
         U1[r] = up;
-        Ly[r] = lam;
         Q[g.base + r] = wn * lam;
       }
       if (sr0 >= 0) {
-        // lambda_m at source pixels of the output rows -> dJ/dx[b, m-1]
+        // lambda_m at source pixels of the output rows -> dJ/dx[b, m-1]:
+        // picked with selects, at most two stores
         float *gsrc_row = a.gsrc + ((int64_t)b * a.T + (m - 1)) * a.nsrcslot;
+        float v0 = Ly[0], v1 = Ly[0];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          if (r == sr0 && sl0 >= 0 && ((omask >> r) & 1u)) gsrc_row[sl0] = Ly[r];
-          if (r == sr1 && sl1 >= 0 && ((omask >> r) & 1u)) gsrc_row[sl1] = Ly[r];
+        for (int r = 1; r < R; ++r) {
+          v0 = (r == sr0) ? Ly[r] : v0;
+          v1 = (r == sr1) ? Ly[r] : v1;
         }
+        if (sl0 >= 0 && ((omask >> sr0) & 1u)) gsrc_row[sl0] = v0;
+        if (sl1 >= 0 && sr1 >= 0 && ((omask >> sr1) & 1u)) gsrc_row[sl1] = v1;
       }
     }
+    load_record(m - 1, s + 1 < a.nsteps);
     __syncthreads();
   };
 

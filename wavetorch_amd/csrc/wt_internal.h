@@ -151,6 +151,22 @@ extern __device__ unsigned long long *wt_phase_ts;
   } while (0)
 #endif
 
+// a / b for the nonlinear adjoints' coefficient re-formation (the blocked
+// k_adj_cs_nl and the per-step k_loop_adj_nl_plain use it alike): the
+// hardware reciprocal estimate (1 ulp) and one residual correction
+// (Markstein, "IA-64 and Elementary Functions", 2000: q = a r, e = a - b q,
+// q + e r), within about half an ulp of the IEEE quotient for the
+// normal-range operands met there (1 + q^2, its square, dt^-2 + b dt^-1, the
+// threshold uth).  Four instructions instead of the ten of the IEEE
+// sequence; the forward kernels keep IEEE division (their results are
+// compared bit for bit).
+__device__ __forceinline__ float adj_div(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  const float e = __builtin_fmaf(-b, q, a);
+  return __builtin_fmaf(e, r, q);
+}
+
 // XCD-aware work-item order.  Workgroups are dispatched round-robin over the
 // 8 XCDs of the chip (workgroup i runs on XCD i % 8); remap so that
 // consecutive work items (the strips of one sample, which exchange halos

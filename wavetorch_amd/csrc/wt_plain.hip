@@ -343,10 +343,10 @@ __global__ void k_loop_adj_nl_plain(AdjArgs a, const int32_t *__restrict__ spmap
   const float rho = a.rho[q];
   float bb = a.bgeom[q], cc = a.clin[q], sd = 0.0f, c2raw = 0.0f, dbdu = 0.0f, dcdu = 0.0f;
   if (NL & 1) {
-    const float r = u / a.sat_uth, d = 1.0f + r * r;
-    sd = a.sat_b0 / d;
+    const float r = adj_div(u, a.sat_uth), d = 1.0f + r * r;
+    sd = adj_div(a.sat_b0, d);
     bb += rho * sd;
-    dbdu = rho * a.sat_b0 * (-2.0f * r / a.sat_uth) / (d * d);
+    dbdu = adj_div(rho * a.sat_b0 * adj_div(-2.0f * r, a.sat_uth), d * d);
   }
   if (NL & 2) {
     c2raw = a.c_nl * (u * u);
@@ -359,7 +359,7 @@ __global__ void k_loop_adj_nl_plain(AdjArgs a, const int32_t *__restrict__ spmap
     if (cc > a.cmax) { cc = a.cmax; clamped = true; }
   }
   const float bd = bb * a.dtm1;
-  const float A = 1.0f / (a.dtm2 + bd);
+  const float A = adj_div(1.0f, a.dtm2 + bd);
   const float Bm = a.dtm2 - bd;
   const float v = A * ((a.s2 * u - Bm * uprev) + (cc * cc) * lapu);
   const float dvdb = A * a.dtm1 * (uprev - v);
