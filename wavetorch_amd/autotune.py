@@ -47,13 +47,13 @@ def _store(d):
     os.replace(tmp, CACHE)
 
 
-def key(Nx, Ny, B, T, K, arch=None):
-    return f"{arch or _arch()}:{Nx}x{Ny}:B{B}:T{T}:K{K}"
+def key(Nx, Ny, B, T, K, arch=None, forward_only=False):
+    return f"{arch or _arch()}:{Nx}x{Ny}:B{B}:T{T}:K{K}" + (":fwd" if forward_only else "")
 
 
-def lookup(Nx, Ny, B, T, K=64, arch="gfx950"):
+def lookup(Nx, Ny, B, T, K=64, arch="gfx950", forward_only=False):
     with _lock:
-        e = _load().get(key(Nx, Ny, B, T, K, arch))
+        e = _load().get(key(Nx, Ny, B, T, K, arch, forward_only))
     return dict(e["tuning"]) if e else None
 
 
@@ -61,7 +61,8 @@ def cached(workload: str):
     """Tuning for a named workload of wavetorch_amd.workloads, or None."""
     shapes = {"train_150_b64": (150, 150, 64, 3000, 64),
               "propagate_100_b4": (100, 100, 4, 1000, 64),
-              "nonlinear_150_b32": (150, 150, 32, 3000, 64)}
+              "nonlinear_150_b32": (150, 150, 32, 3000, 64),
+              "scatter_1024": (1024, 1024, 1, 4000, 64, "gfx950", True)}
     if workload not in shapes:
         return None
     return lookup(*shapes[workload])
@@ -90,10 +91,15 @@ def _from_tuning(t):
             "a_tiles": t["a_strips"], "a_tblock": t["a_tblock"], "a_cpt": t["a_cpt"]}
 
 
-def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2):
+def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2,
+         forward_only=False, reps=2):
     """Coordinate search over the tuning fields of AXES for ``model`` on
     batch ``x``, timing a full training step (forward, loss, checkpointed
-    adjoint) on the GPU.  Starts from the library's own choice, tries every
+    adjoint) on the GPU, or with `forward_only` the forward without gradients
+    (fields every ``model.field_stride`` steps when the model has no probes).
+    Each candidate's time is the best of `reps` runs of `steps` steps (a
+    single run can carry a host-side hiccup as large as the differences
+    between good plans).  Starts from the library's own choice, tries every
     option of one field at a time (the rest fixed), keeps any improvement and
     repeats for `rounds` passes or until `budget_s`; caches the winner under
     key(Nx, Ny, B, T, K) and returns it."""
@@ -104,6 +110,10 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
     t_start = time.perf_counter()
 
     def step():
+        if forward_only:
+            with torch.no_grad():
+                model(x)
+            return
         model.zero_grad(set_to_none=True)
         y = model(x)
         if labels is None:
@@ -117,16 +127,18 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
         try:
             plan = model.plan(B, T, x.device)
             step()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                step()
-            torch.cuda.synchronize()
+            t = float("inf")
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    step()
+                torch.cuda.synchronize()
+                t = min(t, (time.perf_counter() - t0) / steps)
         except (ValueError, RuntimeError) as e:  # a tiling the variants cannot hold
             if verbose:
                 print("skip", tun, str(e)[:120], flush=True)
             return None, float("inf")
-        t = (time.perf_counter() - t0) / steps
         if verbose:
             print(f"{ {k: v for k, v in plan.tuning.items() if v} } {t * 1e3:.2f} ms", flush=True)
         return plan.tuning, t
@@ -149,17 +161,28 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
             return True
         return False
 
+    # tile counts around the library's own (large grids start far above the
+    # fixed options)
+    axes = dict(AXES)
+    for f in ("tiles", "a_tiles"):
+        g0 = state[f]
+        axes[f] = tuple(sorted(set(AXES[f]) | {max(1, round(g0 * r)) for r in
+                                               (0.5, 0.625, 0.75, 0.875, 1.125, 1.25, 1.5)}))
+    if forward_only:  # the adjoint's fields do not take part
+        axes = {f: o for f, o in axes.items() if not f.startswith("a_")}
     for _ in range(rounds):
         improved = False
-        for field, options in AXES.items():
+        for field, options in axes.items():
             for v in options:
                 improved |= consider(dict(state, **{field: v}))
         # block depth and rows per thread are coupled (deeper blocks need more
         # rows per thread to fit): also move them together
-        for f1, f2 in (("tblock", "cpt"), ("a_tblock", "a_cpt")):
+        pairs = (("tblock", "cpt"),) if forward_only else (("tblock", "cpt"),
+                                                           ("a_tblock", "a_cpt"))
+        for f1, f2 in pairs:
             base_state = dict(state)
-            for v1 in AXES[f1]:
-                for v2 in AXES[f2]:
+            for v1 in axes[f1]:
+                for v2 in axes[f2]:
                     improved |= consider(dict(base_state, **{f1: v1, f2: v2}))
         if not improved:
             break
@@ -167,7 +190,8 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
     model.tuning = best
     with _lock:
         d = _load()
-        d[key(Nx, Ny, B, T, K)] = {"tuning": best, "ms": best_t * 1e3}
+        d[key(Nx, Ny, B, T, K, forward_only=forward_only)] = {"tuning": best,
+                                                              "ms": best_t * 1e3}
         _store(d)
     return best
 
