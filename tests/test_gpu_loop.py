@@ -106,3 +106,37 @@ def test_train_shape_bitwise_and_grad(cuda):
         g = co.adjoint(b, c, x, 0.707, 1.0, p["src"], p["taps"], ybar=w)
         assert rel(ct.grad.cpu().numpy(), g["gc"]) < 1e-4, plan.tuning
         assert rel(xt.grad.cpu().numpy(), g["gx"]) < 1e-4, plan.tuning
+
+
+EDGE = [
+    # (Nx, Ny, B, T, K, tuning)
+    (40, 36, 1, 1, 32, {}),                                   # one step
+    (40, 36, 2, 2, 32, {}),
+    (40, 36, 2, 3, 32, {"strips": 2, "tcols": 2, "tblock": 8, "nt": 1024, "cpt": 8}),
+    (40, 36, 2, 20, 64, {}),                                  # K > T
+    (40, 36, 3, 97, 32, {}),                                  # T = 3K + 1
+    (40, 36, 3, 97, 32, {"plain": 1}),
+    (40, 36, 2, 10, 1, {}),                                   # a checkpoint every step
+    (23, 19, 2, 60, 16, {"strips": 3, "tcols": 3, "tblock": 12, "nt": 1024, "cpt": 4}),
+]
+
+
+@pytest.mark.parametrize("Nx,Ny,B,T,K,tun", EDGE,
+                         ids=[f"{e[0]}x{e[1]}-B{e[2]}-T{e[3]}-K{e[4]}-{'-'.join(map(str, e[5].values())) or 'auto'}"
+                              for e in EDGE])
+def test_edge_sizes(Nx, Ny, B, T, K, tun, cuda):
+    """One to three steps, K larger than T, T not a multiple of K, a
+    checkpoint every step, batch 1, and a small odd grid whose tiles are
+    narrower than the time-block halo: forward bitwise, gradients within 1e-4
+    of the fp64 oracle."""
+    p = small_problem(Nx=Nx, Ny=Ny, B=B, T=T)
+    print("case", Nx, Ny, B, T, K, tun, flush=True)
+    plan, x, c, y = _run(p, tun, cuda, grad=True, K=K)
+    yo, _ = co.forward(p["b"], p["c"], p["x"], p["dt"], p["h"], p["src"], p["taps"])
+    assert y.shape == (B, T, 3)
+    assert np.array_equal(y.detach().cpu().numpy(), yo), plan.tuning
+    w = np.random.default_rng(T).standard_normal(y.shape).astype(np.float32)
+    (y * torch.from_numpy(w).to(cuda)).sum().backward()
+    g = co.adjoint(p["b"], p["c"], p["x"], p["dt"], p["h"], p["src"], p["taps"], ybar=w)
+    assert rel(c.grad.cpu().numpy(), g["gc"]) < 1e-4, plan.tuning
+    assert rel(x.grad.cpu().numpy(), g["gx"]) < 1e-4, plan.tuning
