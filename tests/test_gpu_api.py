@@ -211,3 +211,31 @@ def test_wavernn_holey_geometry(cuda):
     mg = model.cell.geom
     for k in ("x", "y", "r"):
         assert rel(getattr(mg, k).grad.cpu().numpy(), getattr(g, k).grad.numpy()) < 1e-4, k
+
+
+def test_training_step_does_not_synchronise(cuda):
+    """After the first step (plans built, host copies of the geometry scalars
+    taken), a training step -- rho -> c, the loop, the loss, the checkpointed
+    adjoint, Adam -- makes no synchronising torch call, so the host keeps
+    enqueueing ahead of the device."""
+    from wavetorch_amd import workloads
+    from wavetorch_amd.utils import normalize_power
+    model, x, labels = workloads.propagate_100_b4(T=64)
+    model = model.to(cuda)
+    x, labels = x.to(cuda), labels.to(cuda)
+    opt = torch.optim.Adam(model.parameters(), lr=1e-3)
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        y = model(x)
+        torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels).backward()
+        opt.step()
+
+    step()
+    torch.cuda.synchronize()
+    torch.cuda.set_sync_debug_mode("error")
+    try:
+        step()
+    finally:
+        torch.cuda.set_sync_debug_mode(0)
+    torch.cuda.synchronize()

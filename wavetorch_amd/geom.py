@@ -265,11 +265,25 @@ class WaveGeometryFreeForm(WaveGeometry):
         c0, c1 = self.c0.item(), self.c1.item()
         return (self.c - c0) / (c1 - c0)
 
+    def _host_params(self):
+        """(blur_N, eta, beta, c0, c1) as host numbers, re-read from the
+        device buffers only when one of them was replaced or modified
+        (data_ptr, _version): reading them every step would synchronise the
+        host with the device once per training step."""
+        ts = (self.blur_N, self.eta, self.beta, self.c0, self.c1)
+        key = tuple((t.data_ptr(), t._version) for t in ts)
+        cached = getattr(self, "_host_cache", None)
+        if cached is None or cached[0] != key:
+            vals = (int(self.blur_N.item()), float(self.eta.item()), float(self.beta.item()),
+                    float(self.c0.item()), float(self.c1.item()))
+            cached = (key, vals)
+            self._host_cache = cached
+        return cached[1]
+
     @property
     def c(self):
         """c0 + (c1-c0) * P(Blur^N(rho)) (geom.py:231-233), gfx950 kernels."""
         _require_hip(self.rho, "WaveGeometryFreeForm.c")
         k = self.blur_kernel[0, 0].to(device=self.rho.device, dtype=torch.float32)
-        return _FreeFormC.apply(self.rho, k, self._blur_radius, int(self.blur_N.item()),
-                                float(self.eta.item()), float(self.beta.item()),
-                                float(self.c0.item()), float(self.c1.item()))
+        blur_N, eta, beta, c0, c1 = self._host_params()
+        return _FreeFormC.apply(self.rho, k, self._blur_radius, blur_N, eta, beta, c0, c1)

@@ -124,8 +124,7 @@ class WaveRNN(torch.nn.Module):
         out = WaveLoop.apply(x, c, b, rho, plan, fields_mode, stride)
         if fields_mode:
             return out
-        L = len(self.probes[0].taps()) if len(self.probes) else 1
-        shp = self.probes[0].out_shape()
+        shp = self.probes[0].out_shape()  # from tensor shapes: no device read
         if len(shp) == 0:
             return out
         return out.reshape(B, T, *shp, len(self.probes))
