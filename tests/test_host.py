@@ -48,6 +48,34 @@ def test_last_error_is_nul_terminated():
     assert n > 8 and len(buf.value) == 7
 
 
+def test_entry_points_reject_null_arguments():
+    """Every compute and query entry point checks its arguments before any
+    device call: null plans, buffers or descriptors give WT_EINVAL (-1) and a
+    message naming the entry point (include/wavetorch_hip.h status codes)."""
+    import ctypes
+    L = _lib.lib()
+    names = ["wt_coef_linear", "wt_step_fwd", "wt_step_bwd", "wt_plan_resolve",
+             "wt_plan_create", "wt_plan_tuning", "wt_workspace_bytes", "wt_loop_fwd",
+             "wt_loop_adj", "wt_geom_freeform_c", "wt_geom_freeform_c_bwd",
+             "wt_batch_plan_create", "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
+             "wt_batch_fwd", "wt_batch_adj", "wt_batch_fwd_adj", "wt_graph_launch"]
+    for n in names:
+        argtypes = _lib._SIGS[n][1]
+        args = []
+        for t in argtypes:
+            if t in (ctypes.c_float, ctypes.c_double):
+                args.append(1.0)
+            elif t in (ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t):
+                args.append(1)
+            else:
+                args.append(None)
+        rc = getattr(L, n)(*args)
+        assert rc == -1, (n, rc)
+        buf = ctypes.create_string_buffer(256)
+        L.wt_last_error(buf, 256)
+        assert buf.value.decode().startswith(n), (n, buf.value)
+
+
 def c2_cfg(N=150):
     return LoopConfig(Nx=N, Ny=N, dt=0.707, h=1.0, src_px=((40, 75, 0),),
                       taps=((110, 55, 1), (110, 75, 1), (110, 95, 1)))

@@ -747,7 +747,13 @@ int wt_plan_resolve(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
                     const wt_tuning *tuning, wt_tuning *out, int64_t *saved, int64_t *temp) {
   Plan *p = nullptr;
   int rc = plan_build(cfg, B, T, cmax_geom, tuning, &p);
-  if (rc) return rc;
+  if (rc) {
+    // the plan builder's messages name wt_plan_create; name this entry point
+    const std::string create = "wt_plan_create";
+    if (g_last_error.compare(0, create.size(), create) == 0)
+      g_last_error = "wt_plan_resolve" + g_last_error.substr(create.size());
+    return rc;
+  }
   if (out) *out = p->tun;
   if (saved) *saved = (int64_t)p->saved_bytes;
   if (temp) *temp = (int64_t)p->temp_bytes;
