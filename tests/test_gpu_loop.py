@@ -5,7 +5,7 @@ contraction, so probe series and fields must equal the C oracle's fp32 loop
 (wave_oracle.c, itself bitwise equal to the torch restatement of the
 reference) bit for bit, for the plain and every tiled variant.
 Adjoint: gradients wrt c and x must match the fp64 oracle adjoint within
-rel 1e-4 (BASELINE.json tolerance for gradients).
+rel 1e-4 (the gradient tolerance of the path).
 """
 import numpy as np
 import pytest
