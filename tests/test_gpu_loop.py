@@ -140,3 +140,63 @@ def test_edge_sizes(Nx, Ny, B, T, K, tun, cuda):
     g = co.adjoint(p["b"], p["c"], p["x"], p["dt"], p["h"], p["src"], p["taps"], ybar=w)
     assert rel(c.grad.cpu().numpy(), g["gc"]) < 1e-4, plan.tuning
     assert rel(x.grad.cpu().numpy(), g["gx"]) < 1e-4, plan.tuning
+
+
+def test_loop_graph_replay(cuda):
+    """wt_loop_fwd + wt_loop_adj captured once (wt_graph_begin/end on a side
+    stream) and replayed three times with new c and x written into the same
+    buffers, eager runs in between: every replay equals the eager result
+    bit for bit (the loop records no memset nodes: buffers are cleared by
+    fill kernels)."""
+    import ctypes
+    from wavetorch_amd import _lib
+    L = _lib.lib()
+    p = small_problem(B=2, T=90)
+    cfg = _cfg(p)
+    plan = Plan(cfg, p["B"], p["T"], 1.0, cuda, {"ck_every": 32})
+    B, T, ntap = p["B"], p["T"], len(cfg.taps)
+    b = torch.from_numpy(p["b"]).to(cuda)
+    c = torch.from_numpy(p["c"]).to(cuda)
+    x = torch.from_numpy(p["x"]).to(cuda)
+    ybar = torch.from_numpy(np.random.default_rng(3).standard_normal((B, T, ntap))
+                            .astype(np.float32)).to(cuda)
+    y = torch.empty((B, T, ntap), device=cuda)
+    saved = torch.empty(plan.saved_bytes, dtype=torch.uint8, device=cuda)
+    temp = torch.empty(plan.temp_bytes, dtype=torch.uint8, device=cuda)
+    gc = torch.empty_like(c)
+    gx = torch.empty_like(x)
+    s = torch.cuda.Stream(device=cuda)
+
+    def run(stream):
+        sp = stream.cuda_stream
+        _lib.check(L.wt_loop_fwd(plan.handle, b.data_ptr(), c.data_ptr(), None, x.data_ptr(),
+                                 y.data_ptr(), None, 1, 1, saved.data_ptr(), temp.data_ptr(),
+                                 sp), "wt_loop_fwd")
+        _lib.check(L.wt_loop_adj(plan.handle, b.data_ptr(), c.data_ptr(), None, x.data_ptr(),
+                                 ybar.data_ptr(), None, saved.data_ptr(), temp.data_ptr(),
+                                 gc.data_ptr(), None, None, gx.data_ptr(), sp), "wt_loop_adj")
+
+    graph = None
+    try:
+        with torch.cuda.stream(s):
+            for rep in range(3):
+                c.copy_(torch.from_numpy(p["c"]).to(cuda) * (1.0 - 0.04 * rep))
+                x.copy_(torch.from_numpy(p["x"]).to(cuda) * (1.0 + 0.5 * rep))
+                run(s)  # eager
+                s.synchronize()
+                ref = (y.clone(), gc.clone(), gx.clone())
+                if graph is None:
+                    _lib.check(L.wt_graph_begin(s.cuda_stream), "wt_graph_begin")
+                    run(s)
+                    g = ctypes.c_void_p()
+                    _lib.check(L.wt_graph_end(s.cuda_stream, ctypes.byref(g)), "wt_graph_end")
+                    graph = g
+                for t in (y, gc, gx):
+                    t.fill_(float("nan"))
+                _lib.check(L.wt_graph_launch(graph, s.cuda_stream), "wt_graph_launch")
+                s.synchronize()
+                assert torch.equal(y, ref[0]) and torch.equal(gc, ref[1]) and \
+                    torch.equal(gx, ref[2]), rep
+    finally:
+        if graph is not None:
+            L.wt_graph_destroy(graph)
