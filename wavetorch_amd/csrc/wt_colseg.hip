@@ -386,7 +386,7 @@ constexpr int CS_MAX_SEEDS = 16;
 // values (up/centre/down neighbours) are re-formed as KW*Lc -- the same fp32
 // product the previous step stored to LDS -- instead of being kept in R more
 // registers.
-template <int R, bool SP>
+template <int R, int NSP, bool GS>
 __device__ __forceinline__ void cs_adj_rows(
     const float *__restrict__ P, float *__restrict__ Q, float (&Lc)[R], float (&Ln)[R],
     const float (&KA1)[R], const float (&KW)[R], int base, int CS, uint32_t omask, int sr0,
@@ -404,28 +404,26 @@ __device__ __forceinline__ void cs_adj_rows(
     lapw = lapw + dn;
     // a1*L1 + lapw - (a1-1)*L2 = a1*(L1 - L2) + (lapw + L2)
     float lam = __builtin_fmaf(KA1[r], Lc[r] - Ln[r], lapw + Ln[r]);
-    if (SP) {
-      if (r == sr0) lam += gs0;
-      if (r == sr1) lam += gs1;
-    }
+    if (NSP >= 1 && r == sr0) lam += gs0;
+    if (NSP >= 2 && r == sr1) lam += gs1;
     upv = ct;
     ct = dn;
     Ln[r] = lam;
     Q[base + r] = KW[r] * lam;
   }
-  if (SP) {
+  if (GS) {
     // lambda_m at source pixels of the output rows -> dJ/dx[b, m-1]: pick
-    // the two rows' values with selects, then at most two stores (one
-    // masked store per row would put R branches on every step of the
-    // waves that hold special pixels, which the step's barrier waits for)
+    // the rows' values with selects, then at most NSP stores (one masked
+    // store per row would put R branches on every step of the waves that
+    // hold special pixels, which the step's barrier waits for)
     float v0 = Ln[0], v1 = Ln[0];
 #pragma unroll
     for (int r = 1; r < R; ++r) {
       v0 = (r == sr0) ? Ln[r] : v0;
-      v1 = (r == sr1) ? Ln[r] : v1;
+      if (NSP >= 2) v1 = (r == sr1) ? Ln[r] : v1;
     }
     if (sl0 >= 0 && sr0 >= 0 && ((omask >> sr0) & 1u)) gsrc_row[sl0] = v0;
-    if (sl1 >= 0 && sr1 >= 0 && ((omask >> sr1) & 1u)) gsrc_row[sl1] = v1;
+    if (NSP >= 2 && sl1 >= 0 && sr1 >= 0 && ((omask >> sr1) & 1u)) gsrc_row[sl1] = v1;
   }
 }
 
@@ -532,7 +530,11 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
       else { sr1 = rr; sd1 = seed_slot; sl1 = src_slot; }
     }
   }
+  // wave-uniform special-pixel path: none, one slot, two slots; source
+  // gradient stores only in waves that hold a source pixel
   const bool wave_sp = __ballot(sr0 >= 0) != 0;
+  const bool wave_sp2 = __ballot(sr1 >= 0) != 0;
+  const bool wave_gs = __ballot(sl0 >= 0 || sl1 >= 0) != 0;
   // output cells of this thread (gradient write-back, dJ/dx)
   uint32_t omask = 0;
   if (g.active && g.C0 + g.c >= g.oc0 && g.C0 + g.c < g.oc1) {
@@ -580,12 +582,20 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
     if (need) {
       if (wave_sp) {
         const float gs0 = sd0 >= 0 ? seeds[sd0 * a.nsteps + s] : 0.0f;
-        const float gs1 = sd1 >= 0 ? seeds[sd1 * a.nsteps + s] : 0.0f;
-        cs_adj_rows<R, true>(P, Q, Lc, Ln, KA1, KW, g.base, g.CS, omask, sr0, sr1, gs0, gs1,
-                             sl0, sl1, gsrc_row);
+        if (wave_sp2) {
+          const float gs1 = sd1 >= 0 ? seeds[sd1 * a.nsteps + s] : 0.0f;
+          cs_adj_rows<R, 2, true>(P, Q, Lc, Ln, KA1, KW, g.base, g.CS, omask, sr0, sr1, gs0,
+                                  gs1, sl0, sl1, gsrc_row);
+        } else if (wave_gs) {
+          cs_adj_rows<R, 1, true>(P, Q, Lc, Ln, KA1, KW, g.base, g.CS, omask, sr0, -1, gs0,
+                                  0.f, sl0, -1, gsrc_row);
+        } else {
+          cs_adj_rows<R, 1, false>(P, Q, Lc, Ln, KA1, KW, g.base, g.CS, omask, sr0, -1, gs0,
+                                   0.f, -1, -1, gsrc_row);
+        }
       } else {
-        cs_adj_rows<R, false>(P, Q, Lc, Ln, KA1, KW, g.base, g.CS, omask, -1, -1, 0.f, 0.f,
-                              -1, -1, gsrc_row);
+        cs_adj_rows<R, 0, false>(P, Q, Lc, Ln, KA1, KW, g.base, g.CS, omask, -1, -1, 0.f, 0.f,
+                                 -1, -1, gsrc_row);
       }
     }
     // gradient sum, then refill SG with lap(u) of step m-2 (the record is
