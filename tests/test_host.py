@@ -97,6 +97,47 @@ def test_resolve_training_shape_heuristic():
     assert saved < math.ceil(3000 / 64) * 2 * 64 * F * 4 + 8 * 2 ** 20
 
 
+def nl_cfg():
+    return LoopConfig(Nx=150, Ny=150, dt=0.64, h=1.0, src_px=((40, 75, 0),),
+                      taps=((110, 55, 1), (110, 75, 1), (110, 95, 1)),
+                      sat_b0=0.5, sat_uth=math.sqrt(0.1), c_nl=0.05, c_clamp=(0.4, 1.1))
+
+
+def test_resolve_rectangular_tilings():
+    """The heuristic takes the fewest tiles per sample that give 256
+    workgroups, near-square tiles first: the nonlinear batch-32 step gets 4x2
+    tiles with 8-step forward blocks at 6 rows per thread and 5-step adjoint
+    blocks (the deepest that fit 4 rows per thread); a 300x100 grid gets 8x2
+    tiles of 37x50; batch 4 on 100x100 stays at 3x3, the most tiles whose own
+    rows are twice the 16-step halo."""
+    t, _, _ = resolve(nl_cfg(), 32, 3000, 1.0)
+    assert (t["strips"], t["tcols"], t["tblock"], t["nt"], t["cpt"]) == (4, 2, 8, 1024, 6)
+    assert (t["a_strips"], t["a_tcols"], t["a_tblock"], t["a_nt"], t["a_cpt"]) == \
+        (4, 2, 5, 1024, 4)
+    rect = LoopConfig(Nx=300, Ny=100, dt=0.707, h=1.0, src_px=((100, 50, 0),),
+                      taps=((200, 50, 1),))
+    t, _, _ = resolve(rect, 16, 1000, 1.0)
+    assert (t["strips"], t["tcols"]) == (8, 2)
+    sq = LoopConfig(Nx=100, Ny=100, dt=0.707, h=1.0, src_px=((20, 50, 0),),
+                    taps=((80, 30, 1), (80, 50, 1), (80, 70, 1)))
+    t, _, _ = resolve(sq, 4, 1000, 1.0)
+    assert (t["strips"], t["tcols"], t["tblock"]) == (3, 3, 16)
+
+
+def test_autotune_tiling_options():
+    from wavetorch_amd import autotune
+    opts = autotune.tilings(150, 150, 4, 2)
+    assert (4, 2) in opts and (2, 4) in opts and (4, 4) in opts and (2, 2) in opts
+    for s, c in opts:  # tiles at most 2:1, tile count within 4x of 8
+        assert max((150 / s) / (150 / c), (150 / c) / (150 / s)) <= 2.0 + 1e-9
+        assert 2 <= s * c <= 32
+    st = {"tiling": (4, 2), "tblock": 8, "cpt": 6, "a_tiling": (2, 4), "a_tblock": 5,
+          "a_cpt": 4}
+    tun = autotune._to_tuning(st)
+    assert (tun["strips"], tun["tcols"], tun["a_strips"], tun["a_tcols"]) == (4, 2, 2, 4)
+    assert autotune._from_tuning(tun) == st
+
+
 def test_resolve_single_tile_runs_whole_segments():
     """One tile per sample has no halo: one launch per checkpoint segment."""
     cfg = LoopConfig(Nx=40, Ny=36, dt=0.707, h=1.0, src_px=((9, 18, 0),), taps=((30, 12, 1),))
@@ -133,9 +174,7 @@ def test_resolve_cfl_and_indices():
 def test_checkpoint_memory_vs_K():
     """Saved + temporary bytes of the nonlinear 150x150 batch-32 loop for K in
     {16, 64, 256}: checkpoints scale as T/K, the segment buffer as K."""
-    cfg = LoopConfig(Nx=150, Ny=150, dt=0.64, h=1.0, src_px=((40, 75, 0),),
-                     taps=((110, 55, 1), (110, 75, 1), (110, 95, 1)),
-                     sat_b0=0.5, sat_uth=math.sqrt(0.1), c_nl=0.05, c_clamp=(0.4, 1.1))
+    cfg = nl_cfg()
     F = 32 * 150 * 150 * 4
     tot = {}
     for K in (16, 64, 256):

@@ -1,6 +1,6 @@
 """Autotuner for the temporally blocked kernels.
 
-``tune`` searches tiles per sample x time-block depth x rows per thread (for
+``tune`` searches tilings per sample x time-block depth x rows per thread (for
 the forward and the adjoint separately) for one problem shape by timing whole
 training steps on the GPU, starting from the library's own plan; ``tune_batch``
 does the same for batched instances.  Winners go to an on-disk cache keyed by
@@ -69,26 +69,49 @@ def cached(workload: str):
 
 
 # Options of each tuning field the search moves through (column-segment
-# family; rows per thread `cpt` must be a compiled variant).
+# family; rows per thread `cpt` must be a compiled variant).  "tiling" and
+# "a_tiling" are (strips, tcols) pairs: s x c tiles per sample for the
+# forward and the adjoint; their options are built per problem (tilings).
 AXES = {
-    "tiles": (1, 2, 3, 4, 5, 6),     # g x g tiles per sample (forward)
+    "tiling": (),
     "tblock": (4, 6, 8, 12, 16, 20, 24, 32),
     "cpt": (4, 6, 8, 10, 12),
-    "a_tiles": (1, 2, 3, 4, 5, 6),   # g x g tiles per sample (adjoint)
-    "a_tblock": (6, 8, 10, 12, 13, 16, 20, 24),
+    "a_tiling": (),
+    "a_tblock": (4, 5, 6, 8, 10, 12, 13, 16, 20, 24),
     "a_cpt": (4, 6, 8, 10),
 }
 
 
+def tilings(Nx, Ny, s0, c0):
+    """(strips, tcols) options around the plan's own s0 x c0: every pair with
+    s, c in 1..6 or within 0.5x-1.5x of s0, c0 whose tiles are at most 2:1
+    and whose tile count is within 4x of s0*c0 either way."""
+    def near(v0):
+        return set(range(1, 7)) | {max(1, round(v0 * r)) for r in
+                                   (0.5, 0.625, 0.75, 0.875, 1.125, 1.25, 1.5)} | {v0}
+    out = []
+    for s in sorted(v for v in near(s0) if v <= Nx):
+        for c in sorted(v for v in near(c0) if v <= Ny):
+            if (s, c) != (s0, c0):
+                if not s0 * c0 / 4 <= s * c <= s0 * c0 * 4:
+                    continue
+                if max((Nx / s) / (Ny / c), (Ny / c) / (Nx / s)) > 2.0 + 1e-9 and s * c > 1:
+                    continue
+            out.append((s, c))
+    return tuple(out)
+
+
 def _to_tuning(st):
-    return {"kind": 0, "strips": st["tiles"], "tcols": st["tiles"], "tblock": st["tblock"],
-            "nt": 1024, "cpt": st["cpt"], "a_strips": st["a_tiles"], "a_tcols": st["a_tiles"],
+    return {"kind": 0, "strips": st["tiling"][0], "tcols": st["tiling"][1],
+            "tblock": st["tblock"], "nt": 1024, "cpt": st["cpt"],
+            "a_strips": st["a_tiling"][0], "a_tcols": st["a_tiling"][1],
             "a_tblock": st["a_tblock"], "a_nt": 1024, "a_cpt": st["a_cpt"]}
 
 
 def _from_tuning(t):
-    return {"tiles": t["strips"], "tblock": t["tblock"], "cpt": t["cpt"],
-            "a_tiles": t["a_strips"], "a_tblock": t["a_tblock"], "a_cpt": t["a_cpt"]}
+    return {"tiling": (t["strips"], t["tcols"]), "tblock": t["tblock"], "cpt": t["cpt"],
+            "a_tiling": (t["a_strips"], t["a_tcols"]), "a_tblock": t["a_tblock"],
+            "a_cpt": t["a_cpt"]}
 
 
 def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2,
@@ -161,13 +184,11 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
             return True
         return False
 
-    # tile counts around the library's own (large grids start far above the
-    # fixed options)
+    # tilings around the library's own (large grids start far above the
+    # small counts)
     axes = dict(AXES)
-    for f in ("tiles", "a_tiles"):
-        g0 = state[f]
-        axes[f] = tuple(sorted(set(AXES[f]) | {max(1, round(g0 * r)) for r in
-                                               (0.5, 0.625, 0.75, 0.875, 1.125, 1.25, 1.5)}))
+    for f in ("tiling", "a_tiling"):
+        axes[f] = tilings(Nx, Ny, *state[f])
     if forward_only:  # the adjoint's fields do not take part
         axes = {f: o for f, o in axes.items() if not f.startswith("a_")}
     for _ in range(rounds):

@@ -220,6 +220,28 @@ static void build_tiles(Plan &p, int strips, int cols, int halo) {
   p.ntiles = strips * cols;
 }
 
+// Heuristic tilings of an Nx x Ny grid (s strips x c columns of tiles per
+// sample): by tile count, then by how far the tiles are from square
+// (|log2 of (Nx/s)/(Ny/c)|), then more strips first.  Tilings whose tiles
+// are more than 2:1 are left out, except the single tile.
+static std::vector<std::pair<int, int>> tile_order(int Nx, int Ny) {
+  struct Cand { int s, c; double bad; };
+  std::vector<Cand> v;
+  for (int s = 1; s <= std::min(Nx, 128); ++s)
+    for (int c = 1; c <= std::min(Ny, 128); ++c) {
+      const double bad = std::fabs(std::log2(((double)Nx / s) / ((double)Ny / c)));
+      if (s * c == 1 || bad <= 1.0 + 1e-9) v.push_back({s, c, bad});
+    }
+  std::stable_sort(v.begin(), v.end(), [](const Cand &a, const Cand &b) {
+    if (a.s * a.c != b.s * b.c) return a.s * a.c < b.s * b.c;
+    if (std::fabs(a.bad - b.bad) > 1e-9) return a.bad < b.bad;
+    return a.s > b.s;
+  });
+  std::vector<std::pair<int, int>> out;
+  for (auto &e : v) out.emplace_back(e.s, e.c);
+  return out;
+}
+
 // Checks that the tiled kernels can take this tiling: region fits the
 // (nt, cpt) variant and LDS, per-thread source/special/tap slots suffice.
 static bool tiles_ok(const Plan &p, int nt, int cpt) {
@@ -565,22 +587,27 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
   if (tiled) {
     int strips = tun.strips, tb = tun.tblock, nt = tun.nt, cpt = tun.cpt;
     // the nonlinear cell forms its coefficients (three divisions) per cell
-    // and step: halo work costs more there, so shallower blocks and smaller
-    // tiles (two workgroups per CU) pay
+    // and step: halo work costs more there, so shallower blocks pay
     if (tb <= 0) tb = p->nl ? 8 : 16;
     if (tb > p->K) tb = p->K;
     if (tun.kind == 0 && tb > 256) tb = 256;  // inputs/seeds of a launch are staged in LDS
-    // Heuristic for the column-segment family: g x g square tiles per
-    // sample, g the smallest giving one workgroup per CU (B*g*g >= 256; 512
-    // for the nonlinear cell) while a tile's own rows stay at least twice the
-    // halo; grown further only if no block variant holds the tile.
+    // Heuristic for the column-segment family: s x c tiles per sample, the
+    // first in tile_order's order giving one workgroup per CU (B*s*c >= 256)
+    // among the tilings whose own rows and columns are at least twice the
+    // halo (else the last of those); further along the order only if no
+    // block variant holds the tiles.  (Measured on the nonlinear 150x150
+    // batch-32 step: 4x2 tiles, 256 workgroups, beat 4x4 square tiles, 512,
+    // by 17%.)
     bool square = tun.kind == 0 && strips <= 0 && tun.tcols <= 0;
+    std::vector<std::pair<int, int>> order;
+    size_t oi = 0;
     if (square) {
-      const int64_t target = p->nl ? 512 : 256;
-      strips = 1;
-      while ((int64_t)B * strips * strips < target &&
-             std::min(p->Nx, p->Ny) / (strips + 1) >= 2 * tb)
-        ++strips;
+      order = tile_order(p->Nx, p->Ny);
+      for (size_t i = 1; i < order.size(); ++i) {
+        if ((int64_t)B * order[oi].first * order[oi].second >= 256) break;
+        if (std::min(p->Nx / order[i].first, p->Ny / order[i].second) >= 2 * tb) oi = i;
+      }
+      strips = order[oi].first;
     } else if (strips <= 0) {
       // row strips: enough workgroups to cover the 256 CUs
       strips = 1;
@@ -588,8 +615,11 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
     }
     int tcols = (tun.kind == 0 && tun.tcols > 0) ? tun.tcols : 1;
     bool ok = false;
-    for (; strips <= p->Nx; ++strips) {
-      if (square) tcols = std::min(strips, p->Ny);
+    for (; square ? oi < order.size() : strips <= p->Nx; square ? (void)++oi : (void)++strips) {
+      if (square) {
+        strips = order[oi].first;
+        tcols = order[oi].second;
+      }
       build_tiles(*p, strips, tcols, tb);
       if (nt > 0 && cpt > 0) {
         ok = tun.kind == 1 ? tiles_ok(*p, nt, cpt) : tiles_ok_cs(*p, nt, cpt);
@@ -680,21 +710,39 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
     int acp = tun.a_cpt > 0 ? tun.a_cpt : tun.cpt;
     bool aok = false;
     if (p->nl) {
-      // nonlinear adjoint (k_adj_cs_nl): 4 rows per thread at 1024 threads,
-      // 8-step blocks unless given; tiles grow until the region fits
+      // nonlinear adjoint (k_adj_cs_nl): 4 rows per thread at 1024 threads.
+      // Unless given, the block is the deepest of at most 8 steps whose
+      // regions fit on the forward's tiling (at least 4 steps: deeper blocks,
+      // fewer launches, each with its set-up); failing that, 8-step blocks
+      // on the first larger tiling in tile_order that fits.
       ant = 1024;
       acp = 4;
-      if (tun.a_tblock <= 0) atb = std::min(8, p->K);
+      auto fits = [&](int q, int qc, int d) {
+        build_tiles(*p, q, qc, d);
+        return tiles_ok_cs(*p, ant, acp) &&
+               cs_nl_shmem_bytes(acp, p->RHmax, p->RWmax) + 1024 <= LDS_LIMIT;
+      };
       const bool fixed = tun.a_strips > 0;
-      for (int q = as; q <= p->Nx && !aok; ++q) {
-        if (fixed && q != as) break;
-        const int qc = fixed ? ac : std::min(q, p->Ny);
-        build_tiles(*p, q, qc, atb);
-        aok = tiles_ok_cs(*p, ant, acp) &&
-              cs_nl_shmem_bytes(acp, p->RHmax, p->RWmax) + 1024 <= LDS_LIMIT;
-        if (aok) {
-          as = q;
-          ac = qc;
+      const int dmax = std::min(8, p->K), dmin = fixed ? 1 : std::min(4, p->K);
+      if (tun.a_tblock > 0) {
+        aok = fits(as, ac, atb);
+      } else {
+        for (int d = dmax; d >= dmin && !aok; --d)
+          if (fits(as, ac, d)) {
+            atb = d;
+            aok = true;
+          }
+        if (!aok) atb = dmax;
+      }
+      if (!aok && !fixed) {
+        for (const auto &t : tile_order(p->Nx, p->Ny)) {
+          if (t.first * t.second <= as * ac) continue;
+          if (fits(t.first, t.second, atb)) {
+            as = t.first;
+            ac = t.second;
+            aok = true;
+            break;
+          }
         }
       }
     } else {
