@@ -928,27 +928,7 @@ __global__ __launch_bounds__(NT) void k_adj_cs_nl(AdjArgs a, NlCarry cy) {
 
 // ------------------------------------------------------------------ launchers
 
-static int cs_max_dyn_lds() {
-  static int v = -1;
-  if (v < 0) {
-    int dev = 0, r = 0;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&r, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-      r = 64 * 1024;
-    v = r;
-  }
-  return v;
-}
-
-// Allows a kernel the largest dynamic LDS the device leaves next to the
-// kernel's own static LDS.
-static hipError_t cs_allow_lds(const void *kfn) {
-  hipFuncAttributes at{};
-  hipError_t e = hipFuncGetAttributes(&at, kfn);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             cs_max_dyn_lds() - (int)at.sharedSizeBytes);
-}
+static int cs_max_dyn_lds() { return max_dyn_lds_dev(); }
 
 #define WT_CS_VARIANTS(X) \
   X(1024, 4)              \
@@ -994,12 +974,9 @@ static hipError_t cs_fwd_dispatch(int nl, bool rec, const FwdArgs &a, int nblock
 #define FT(NL, REC)                                                                   \
   do {                                                                                \
     auto kfn = k_fwd_cs<NT, R, NL, REC>;                                              \
-    static bool done = false;                                                         \
-    if (!done) {                                                                      \
-      hipError_t e = cs_allow_lds((const void *)kfn);                                 \
-      if (e != hipSuccess) return e;                                                  \
-      done = true;                                                                    \
-    }                                                                                 \
+    static std::atomic<uint64_t> done{0};                                             \
+    hipError_t e = allow_dyn_lds((const void *)kfn, done);                            \
+    if (e != hipSuccess) return e;                                                    \
     hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, a);              \
   } while (0)
   if (nl == 0) {
@@ -1015,12 +992,9 @@ template <int NT, int R>
 static hipError_t cs_adj_dispatch(const AdjArgs &a, int nblocks, int nthreads, size_t shmem,
                                   hipStream_t s) {
   auto kfn = k_adj_cs<NT, R>;
-  static bool done = false;
-  if (!done) {
-    hipError_t e = cs_allow_lds((const void *)kfn);
-    if (e != hipSuccess) return e;
-    done = true;
-  }
+  static std::atomic<uint64_t> done{0};
+  hipError_t e = allow_dyn_lds((const void *)kfn, done);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, a);
   return hipGetLastError();
 }
@@ -1071,12 +1045,9 @@ hipError_t launch_adj_cs_nl(int nt, int R, bool gb, const AdjArgs &a, const NlCa
 #define NLK(RR, G)                                                                     \
   do {                                                                                 \
     auto kfn = k_adj_cs_nl<1024, RR, G>;                                               \
-    static bool done = false;                                                          \
-    if (!done) {                                                                       \
-      hipError_t e = cs_allow_lds((const void *)kfn);                                  \
-      if (e != hipSuccess) return e;                                                   \
-      done = true;                                                                     \
-    }                                                                                  \
+    static std::atomic<uint64_t> done{0};                                              \
+    hipError_t e = allow_dyn_lds((const void *)kfn, done);                             \
+    if (e != hipSuccess) return e;                                                     \
     hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, a, cy);           \
   } while (0)
   if (gb) NLK(4, true); else NLK(4, false);
@@ -1092,12 +1063,9 @@ static hipError_t cs_fwd_batch_dispatch(bool rec, const BatchFwd &bl, int nblock
 #define FB(REC)                                                                       \
   do {                                                                                \
     auto kfn = k_fwd_cs_batch<NT, R, 0, REC>;                                         \
-    static bool done = false;                                                         \
-    if (!done) {                                                                      \
-      hipError_t e = cs_allow_lds((const void *)kfn);                                 \
-      if (e != hipSuccess) return e;                                                  \
-      done = true;                                                                    \
-    }                                                                                 \
+    static std::atomic<uint64_t> done{0};                                             \
+    hipError_t e = allow_dyn_lds((const void *)kfn, done);                            \
+    if (e != hipSuccess) return e;                                                    \
     hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, bl);             \
   } while (0)
   if (rec) FB(true); else FB(false);
@@ -1109,12 +1077,9 @@ template <int NT, int R>
 static hipError_t cs_adj_batch_dispatch(const BatchAdj &bl, int nblocks, int nthreads,
                                         size_t shmem, hipStream_t s) {
   auto kfn = k_adj_cs_batch<NT, R>;
-  static bool done = false;
-  if (!done) {
-    hipError_t e = cs_allow_lds((const void *)kfn);
-    if (e != hipSuccess) return e;
-    done = true;
-  }
+  static std::atomic<uint64_t> done{0};
+  hipError_t e = allow_dyn_lds((const void *)kfn, done);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(nthreads), shmem, s, bl);
   return hipGetLastError();
 }

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "../../include/wavetorch_hip.h"
 
 namespace wt {
@@ -183,6 +185,41 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 namespace wt {
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what);
+
+// Largest dynamic LDS a workgroup may request on the current device (cached
+// per device id below 64).
+inline int max_dyn_lds_dev() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  const bool slot = dev >= 0 && dev < 64;
+  int v = slot ? cache[dev].load(std::memory_order_relaxed) : 0;
+  if (v > 0) return v;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+      v <= 0)
+    v = 64 * 1024;
+  if (slot) cache[dev].store(v, std::memory_order_relaxed);
+  return v;
+}
+
+// Lets kernel `kfn` request the dynamic LDS the current device leaves next
+// to the kernel's static LDS.  The attribute is set once per device:
+// `done` (one per kernel) holds a bit per device id below 64; higher ids set
+// it on every call.
+inline hipError_t allow_dyn_lds(const void *kfn, std::atomic<uint64_t> &done) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const uint64_t bit = (dev >= 0 && dev < 64) ? (uint64_t)1 << dev : 0;
+  if (bit && (done.load(std::memory_order_acquire) & bit)) return hipSuccess;
+  hipFuncAttributes at{};
+  e = hipFuncGetAttributes(&at, kfn);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          max_dyn_lds_dev() - (int)at.sharedSizeBytes);
+  if (e == hipSuccess && bit) done.fetch_or(bit, std::memory_order_release);
+  return e;
+}
 }  // namespace wt
 
 #define WT_HIP(call)                                   \

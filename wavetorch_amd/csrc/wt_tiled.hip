@@ -424,17 +424,7 @@ __global__ __launch_bounds__(NT) void k_adj_tiled(AdjArgs a) {
 // ------------------------------------------------------------------ launchers
 
 // Largest dynamic LDS a workgroup may request on the current device.
-static int max_dyn_lds() {
-  static int v = -1;
-  if (v < 0) {
-    int dev = 0, r = 0;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&r, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-      r = 64 * 1024;
-    v = r;
-  }
-  return v;
-}
+static int max_dyn_lds() { return max_dyn_lds_dev(); }
 
 template <int NT, int CPT>
 static hipError_t fwd_dispatch(int nl, bool rec, const FwdArgs &a, int nblocks, size_t shmem,
@@ -442,14 +432,9 @@ static hipError_t fwd_dispatch(int nl, bool rec, const FwdArgs &a, int nblocks, 
 #define FT(NL, REC)                                                                  \
   do {                                                                               \
     auto kfn = k_fwd_tiled<NT, CPT, true, NL, REC>;                                  \
-    static bool attr_done = false;                                                   \
-    if (!attr_done) {                                                                \
-      hipError_t e = hipFuncSetAttribute((const void *)kfn,                          \
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                         max_dyn_lds());                             \
-      if (e != hipSuccess) return e;                                                 \
-      attr_done = true;                                                              \
-    }                                                                                \
+    static std::atomic<uint64_t> done{0};                                            \
+    hipError_t e = allow_dyn_lds((const void *)kfn, done);                           \
+    if (e != hipSuccess) return e;                                                   \
     hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(NT), shmem, s, a);                   \
   } while (0)
   if (nl == 0) {
@@ -464,13 +449,9 @@ static hipError_t fwd_dispatch(int nl, bool rec, const FwdArgs &a, int nblocks, 
 template <int NT, int CPT>
 static hipError_t adj_dispatch(const AdjArgs &a, int nblocks, size_t shmem, hipStream_t s) {
   auto kfn = k_adj_tiled<NT, CPT, true>;
-  static bool attr_done = false;
-  if (!attr_done) {
-    hipError_t e = hipFuncSetAttribute((const void *)kfn,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, max_dyn_lds());
-    if (e != hipSuccess) return e;
-    attr_done = true;
-  }
+  static std::atomic<uint64_t> done{0};
+  hipError_t e = allow_dyn_lds((const void *)kfn, done);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kfn, dim3(nblocks), dim3(NT), shmem, s, a);
   return hipGetLastError();
 }
