@@ -256,6 +256,24 @@ int wt_geom_freeform_c_bwd(const float *grad_c, const float *blur_in,
                            void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Loss head (train.py:61-62 with utils.py:35-36)                            */
+/* ------------------------------------------------------------------------ */
+
+/* J = CrossEntropyLoss()(normalize_power(y.sum(dim=1)), labels), the
+ * reference's training criterion on its probe series (train.py:61-62; the
+ * default mean reduction), and dJ/dI (B,P) with I = y.sum(dim=1): the
+ * gradient of J with respect to y[b,t,k] is grad_I[b,k] for every t.
+ *   y       (B,T,P) fp32 probe series, contiguous; P <= 256
+ *   labels  (B) int64 class indices in [0, P); a sample with any other label
+ *           gets a NaN loss term and NaN gradients (the reference raises)
+ *   loss    one fp32 (device); grad_I (B,P) fp32 (device)
+ *   work    B doubles of device scratch (per-sample losses)
+ * Two kernel launches on `stream`, no allocation, no memset. */
+int wt_loss_power_ce(const float *y, const int64_t *labels, int32_t B, int32_t T,
+                     int32_t P, float *loss, float *grad_I, double *work,
+                     void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Kernel timing (no counterpart in the reference, which has no profiler)    */
 /* ------------------------------------------------------------------------ */
 

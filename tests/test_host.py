@@ -57,7 +57,7 @@ def test_entry_points_reject_null_arguments():
     names = ["wt_coef_linear", "wt_step_fwd", "wt_step_bwd", "wt_plan_resolve",
              "wt_plan_create", "wt_plan_tuning", "wt_workspace_bytes", "wt_loop_fwd",
              "wt_loop_adj", "wt_geom_freeform_c", "wt_geom_freeform_c_bwd",
-             "wt_batch_plan_create", "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
+             "wt_loss_power_ce", "wt_batch_plan_create", "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
              "wt_batch_fwd", "wt_batch_adj", "wt_batch_fwd_adj", "wt_graph_launch"]
     for n in names:
         argtypes = _lib._SIGS[n][1]
@@ -236,6 +236,52 @@ def test_rho_constrained_to_design_region_and_pml():
     m = _model()
     g = m.cell.geom
     assert torch.all(g.rho.detach()[g.b > 0] == 0)
+
+
+def test_constrain_equals_mask_assignments():
+    """geom.py:201-205 zeroes rho by two boolean-mask assignments; the select
+    form gives the same values, NaN and infinities included."""
+    import numpy as np
+    from wavetorch_amd import WaveGeometryFreeForm
+    dr = np.zeros((30, 40), dtype=np.uint8)
+    dr[5:20, 8:30] = 1
+    g = WaveGeometryFreeForm((30, 40), 1.0, c0=1.0, c1=0.5, abs_N=4, abs_sig=1.0, abs_p=2,
+                             rho="rand", design_region=dr)
+    r = torch.rand(30, 40) * 3 - 1
+    r[0, 0] = float("nan")
+    r[10, 10] = float("inf")
+    r[12, 12] = float("nan")
+    with torch.no_grad():
+        g.rho.copy_(r)
+    g.constrain_to_design_region()
+    ref = r.clone()
+    ref[torch.from_numpy(dr) == 0] = 0.0
+    ref[g.b > 0] = 0.0
+    assert torch.equal(torch.nan_to_num(g.rho.detach(), nan=7.0), torch.nan_to_num(ref, nan=7.0))
+    assert torch.isnan(g.rho.detach()[12, 12]) and g.rho.detach()[0, 0] == 0
+
+
+def test_save_load_model_round_trip(tmp_path):
+    """io.py:13-87 file layout: the dict keys of the reference, the WaveRNN
+    state_dict, and a model rebuilt from the last geometry state; with no
+    pandas history the restricted loader reads it."""
+    from wavetorch_amd.io import load_model, save_model
+    m = _model()
+    with torch.no_grad():
+        m.cell.geom.rho.mul_(0.5)
+    save_model(m, "m1", str(tmp_path) + "/", cfg={"dtype": "float32"}, verbose=False)
+    raw = torch.load(str(tmp_path / "m1.pt"), weights_only=True)
+    assert set(raw) == {"model_geom_class_str", "model_state", "history", "history_geom_state",
+                        "cfg"}
+    assert raw["model_geom_class_str"] == "WaveGeometryFreeForm"
+    m2, hist, hgs, cfg = load_model(str(tmp_path / "m1.pt"))
+    assert hist is None and cfg == {"dtype": "float32"} and len(hgs) == 1
+    assert torch.equal(m2.cell.geom.rho.detach(), m.cell.geom.rho.detach())
+    assert float(m2.cell.dt) == float(m.cell.dt)
+    assert [(p.x.item(), p.y.item()) for p in m2.probes] == \
+        [(p.x.item(), p.y.item()) for p in m.probes]
+    assert [(q.x.item(), q.y.item()) for q in m2.sources] == \
+        [(q.x.item(), q.y.item()) for q in m.sources]
 
 
 def test_product_path_has_no_cpu_fallback():

@@ -7,12 +7,13 @@ the geometry parameterisation run in hand-written HIP kernels loaded from
 ``wavetorch_amd/_lib/libwavetorch_hip.so`` through the C ABI declared in
 ``include/wavetorch_hip.h``.
 """
-from . import cell, geom, probe, rnn, source, utils
+from . import cell, geom, io, loss, probe, rnn, source, utils
 from .cell import TimeStep, WaveCell
 from .geom import WaveGeometryFreeForm, WaveGeometryHoley
 from .probe import WaveIntensityProbe, WaveProbe
 from .rnn import WaveRNN
 from .source import WaveLineSource, WaveSource
+from .train import GraphedTrainStep, train
 
 __all__ = ["WaveCell", "WaveGeometryHoley", "WaveGeometryFreeForm", "WaveProbe",
            "WaveIntensityProbe", "WaveRNN", "WaveSource", "WaveLineSource"]

@@ -22,7 +22,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "_lib")
 LIBNAME = "libwavetorch_hip.so"
-SOURCES = ["wt_runtime.hip", "wt_plain.hip", "wt_tiled.hip", "wt_colseg.hip", "wt_geom.hip"]
+SOURCES = ["wt_runtime.hip", "wt_plain.hip", "wt_tiled.hip", "wt_colseg.hip", "wt_geom.hip",
+           "wt_loss.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -254,10 +254,13 @@ class WaveGeometryFreeForm(WaveGeometry):
             raise ValueError('The domain initialization is invalid')
 
     def constrain_to_design_region(self):
-        """rho = 0 outside the design region and wherever b > 0 (geom.py:201-205)."""
+        """rho = 0 outside the design region and wherever b > 0 (geom.py:201-205).
+        One elementwise select in place of the reference's two boolean-mask
+        assignments (same values; a mask assignment on the device reads the
+        mask's count back to the host, which also bars graph capture)."""
         with torch.no_grad():
-            self.rho[self.design_region == 0] = 0.0
-            self.rho[self.b > 0] = 0.0
+            keep = (self.design_region != 0) & ~(self.b > 0)
+            self.rho.copy_(torch.where(keep, self.rho, 0.0))
 
     def _rho_model(self):
         """Blurred and projected density P(Blur^N(rho)) (geom.py:224-229); the
