@@ -114,20 +114,44 @@ def build(dev, tuning=None, workload="train_150_b64", T=None):
     return model, x.to(dev), (labels.to(dev) if labels is not None else None)
 
 
-def train_step(model, x, labels, opt):
+def train_step(model, x, labels, opt, fused_loss=False):
+    """One training step: forward, the reference's criterion
+    cross_entropy(normalize_power(y.sum(dim=1)), labels) (train.py:61-62) --
+    torch ops, or the fused loss-head kernel with `fused_loss` -- the
+    checkpointed adjoint and an Adam step."""
+    from wavetorch_amd.loss import power_cross_entropy
     from wavetorch_amd.utils import normalize_power
     if labels is None:  # forward-only workload: snapshot fields, no tape
         with torch.no_grad():
             return model(x, output_fields=True)[0, -1, 0, 0]
     opt.zero_grad(set_to_none=True)
     y = model(x)
-    loss = torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels)
+    if fused_loss:
+        loss = power_cross_entropy(y, labels)
+    else:
+        loss = torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels)
     loss.backward()
     opt.step()
     return loss
 
 
-def time_steps(model, x, labels, steps, warmup, profile=False):
+def time_graph_steps(model, x, labels, steps, warmup):
+    """Times `steps` replays of the training step captured as one HIP graph
+    (wavetorch_amd.GraphedTrainStep: fused loss head, capturable Adam)."""
+    from wavetorch_amd import GraphedTrainStep
+    opt = torch.optim.Adam(model.parameters(), lr=1e-3, capturable=True)
+    st = GraphedTrainStep(model, opt, x, labels, warmup=max(1, warmup), constrain=False)
+    for _ in range(warmup):
+        st(x, labels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = st(x, labels)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, float(loss.item())
+
+
+def time_steps(model, x, labels, steps, warmup, profile=False, fused_loss=False):
     """Times `steps` training steps after `warmup`.  With `profile`, a second
     timed region of the same length runs with the library's HIP-event timing
     of every loop-kernel launch (events recorded on the launch stream); its
@@ -136,12 +160,12 @@ def time_steps(model, x, labels, steps, warmup, profile=False):
     from wavetorch_amd import _lib
     opt = torch.optim.Adam(model.parameters(), lr=1e-3)
     for _ in range(warmup):
-        train_step(model, x, labels, opt)
+        train_step(model, x, labels, opt, fused_loss)
     torch.cuda.synchronize()
     L = _lib.lib()
     t0 = time.perf_counter()
     for _ in range(steps):
-        loss = train_step(model, x, labels, opt)
+        loss = train_step(model, x, labels, opt, fused_loss)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     prof = None
@@ -151,7 +175,7 @@ def time_steps(model, x, labels, steps, warmup, profile=False):
         _lib.check(L.wt_profile_read(None, None, None), "wt_profile_read")
         t0 = time.perf_counter()
         for _ in range(steps):
-            train_step(model, x, labels, opt)
+            train_step(model, x, labels, opt, fused_loss)
         torch.cuda.synchronize()
         dt_p = (time.perf_counter() - t0) / steps
         ms = (ctypes.c_double * 3)()
@@ -445,6 +469,11 @@ def main():
     ap.add_argument("--workload", default="train_150_b64", choices=sorted(WORKLOADS),
                     help="configuration to time (default: the headline training step)")
     ap.add_argument("--tuning", type=str, default=None, help="JSON wt_tuning override")
+    ap.add_argument("--loss", choices=("torch", "fused"), default="torch",
+                    help="loss head of the training step: the reference's torch ops (default) "
+                         "or the fused wt_loss_power_ce kernel")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the step captured as one HIP graph (GraphedTrainStep)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the timed region, copy MIB MiB device-to-device 3 times (the "
                          "known-size copy that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -475,7 +504,9 @@ def main():
     Nx, Ny = model.cell.geom.domain_shape
     cells = B * Nx * Ny * T
     log(f"[bench] {args.workload}: tuning {tuning}")
-    dt, loss, prof = time_steps(model, x, labels, args.steps, args.warmup, profile=True)
+    fused = args.loss == "fused" and labels is not None
+    dt, loss, prof = time_steps(model, x, labels, args.steps, args.warmup, profile=True,
+                                fused_loss=fused)
     plan = next(iter(model._plans.values()))
     log(f"[bench] fused: {dt * 1e3:.2f} ms/step  loss {loss:.4f}  plan {plan.tuning}")
     value = cells / dt / 1e9
@@ -521,9 +552,17 @@ def main():
                           "achieved": round(wl["bytes"] * cells / dt / 1e9, 1), "unit": "GB/s",
                           "frac": round(wl["bytes"] * cells / dt / PEAK_HBM, 4)},
     }
+    if labels is not None:
+        out["loss_head"] = args.loss
     if args.meta_out:
         with open(args.meta_out, "w") as f:
             json.dump({"tuning": plan.tuning, "workload": args.workload}, f)
+    if args.graph and labels is not None:
+        gm, gx, gl = build(dev, tuning, args.workload)
+        gdt, _ = time_graph_steps(gm, gx, gl, args.steps, args.warmup)
+        log(f"[bench] graph: {gdt * 1e3:.2f} ms/step")
+        out["graph_ms_per_step"] = round(gdt * 1e3, 3)
+        del gm, gx, gl
     if args.calib_copy > 0:
         n = args.calib_copy * 2 ** 20 // 4
         a = torch.ones(n, device=dev)

@@ -267,11 +267,16 @@ int wt_geom_freeform_c_bwd(const float *grad_c, const float *blur_in,
  *   labels  (B) int64 class indices in [0, P); a sample with any other label
  *           gets a NaN loss term and NaN gradients (the reference raises)
  *   loss    one fp32 (device); grad_I (B,P) fp32 (device)
- *   work    B doubles of device scratch (per-sample losses)
- * Two kernel launches on `stream`, no allocation, no memset. */
+ *   work    device scratch of wt_loss_power_ce_workspace_bytes(B, T, P) bytes,
+ *           8-byte aligned (partial sums, per-sample losses)
+ * Three kernel launches on `stream`, no allocation, no memset; 1 <= B <= 65535. */
 int wt_loss_power_ce(const float *y, const int64_t *labels, int32_t B, int32_t T,
-                     int32_t P, float *loss, float *grad_I, double *work,
+                     int32_t P, float *loss, float *grad_I, void *work,
                      void *stream);
+
+/* Bytes of `work` that wt_loss_power_ce needs for (B, T, P) (no counterpart
+ * in the reference). */
+int wt_loss_power_ce_workspace_bytes(int32_t B, int32_t T, int32_t P, int64_t *bytes);
 
 /* ------------------------------------------------------------------------ */
 /* Kernel timing (no counterpart in the reference, which has no profiler)    */

@@ -57,7 +57,7 @@ def test_entry_points_reject_null_arguments():
     names = ["wt_coef_linear", "wt_step_fwd", "wt_step_bwd", "wt_plan_resolve",
              "wt_plan_create", "wt_plan_tuning", "wt_workspace_bytes", "wt_loop_fwd",
              "wt_loop_adj", "wt_geom_freeform_c", "wt_geom_freeform_c_bwd",
-             "wt_loss_power_ce", "wt_batch_plan_create", "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
+             "wt_loss_power_ce", "wt_loss_power_ce_workspace_bytes", "wt_batch_plan_create", "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
              "wt_batch_fwd", "wt_batch_adj", "wt_batch_fwd_adj", "wt_graph_launch"]
     for n in names:
         argtypes = _lib._SIGS[n][1]

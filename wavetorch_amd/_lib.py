@@ -86,6 +86,8 @@ _SIGS = {
     "wt_geom_freeform_c_bwd": (ctypes.c_int, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_i32, c_f32,
                                               c_f32, c_f32, c_f32, c_p, c_p, c_p]),
     "wt_loss_power_ce": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p]),
+    "wt_loss_power_ce_workspace_bytes": (ctypes.c_int, [c_i32, c_i32, c_i32,
+                                                        ctypes.POINTER(c_i64)]),
     "wt_batch_plan_create": (ctypes.c_int, [c_i32, ctypes.POINTER(WtConfig),
                                             ctypes.POINTER(c_i32), ctypes.POINTER(c_i32),
                                             ctypes.POINTER(c_f32), ctypes.POINTER(WtTuning),

@@ -12,11 +12,18 @@
 // (p_j = I_j / S: dp_j/dI_k = ([j == k] - p_j) / S), which is dJ/dy[b,t,k]
 // for every t.  Sums and the softmax are formed in double and rounded once.
 //
-// One workgroup per sample reads its T x P series with coalesced loads: a
-// thread of the first floor(256/P)*P threads owns probe tid % P and strides
-// through the flat series by that count, so its loads stay on one probe.
-// A second one-workgroup launch takes the batch mean.  No atomics and no
-// memsets: the pair can be captured in a HIP graph.
+// Three launches, no atomics and no memsets (the sequence can be captured in
+// a HIP graph):
+//  1. partial sums: B x C workgroups, workgroup (c, b) sums steps
+//     [T c / C, T (c+1) / C) of sample b.  A thread of the first
+//     floor(256/P)*P threads owns probe tid % P and strides through the flat
+//     chunk by that count, so its loads stay on one probe and a wave's loads
+//     are contiguous.  C spreads a batch over >= 512 workgroups (at most 16
+//     chunks, at least 64 steps each): a sample per workgroup leaves the
+//     reduction latency-bound on B CUs.
+//  2. per sample: the C partials -> I, the softmax, loss_b and dJ/dI.
+//  3. the batch mean.
+#include <algorithm>
 #include <cmath>
 
 #include "wt_internal.h"
@@ -25,15 +32,23 @@ namespace wt {
 
 constexpr int LOSS_NT = 256;
 constexpr int LOSS_MAX_P = 256;
+constexpr int LOSS_MAX_CHUNKS = 16;
 
-__global__ __launch_bounds__(LOSS_NT) void k_power_ce_sample(
-    const float *__restrict__ y, const int64_t *__restrict__ labels, int64_t T, int P,
-    double invB, float *__restrict__ grad_I, double *__restrict__ loss_b) {
+static int loss_chunks(int B, int T) {
+  int c = (512 + B - 1) / B;
+  c = std::min(c, LOSS_MAX_CHUNKS);
+  c = std::min(c, std::max(1, T / 64));
+  return std::max(c, 1);
+}
+
+__global__ __launch_bounds__(LOSS_NT) void k_power_ce_partial(const float *__restrict__ y,
+                                                              int64_t T, int P,
+                                                              double *__restrict__ part) {
   __shared__ double red[LOSS_NT];
-  __shared__ double pk[LOSS_MAX_P];
-  const int b = blockIdx.x, tid = threadIdx.x;
-  const int64_t n = T * P;
-  const float *ys = y + (int64_t)b * n;
+  const int c = blockIdx.x, C = gridDim.x, b = blockIdx.y, tid = threadIdx.x;
+  const int64_t t0 = T * c / C, t1 = T * (c + 1) / C;
+  const float *ys = y + ((int64_t)b * T + t0) * P;
+  const int64_t n = (t1 - t0) * P;
   const int nact = (LOSS_NT / P) * P;
   double acc = 0.0;
   if (tid < nact)
@@ -43,6 +58,18 @@ __global__ __launch_bounds__(LOSS_NT) void k_power_ce_sample(
   if (tid < P) {
     double s = 0.0;
     for (int j = tid; j < nact; j += P) s += red[j];
+    part[((int64_t)b * C + c) * P + tid] = s;
+  }
+}
+
+__global__ __launch_bounds__(LOSS_NT) void k_power_ce_sample(
+    const double *__restrict__ part, int C, const int64_t *__restrict__ labels, int P,
+    double invB, float *__restrict__ grad_I, double *__restrict__ loss_b) {
+  __shared__ double pk[LOSS_MAX_P];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  if (tid < P) {
+    double s = 0.0;
+    for (int c = 0; c < C; ++c) s += part[((int64_t)b * C + c) * P + tid];
     pk[tid] = s;
   }
   __syncthreads();
@@ -92,22 +119,49 @@ __global__ __launch_bounds__(LOSS_NT) void k_power_ce_mean(const double *__restr
 
 using namespace wt;
 
-extern "C" int wt_loss_power_ce(const float *y, const int64_t *labels, int32_t B, int32_t T,
-                                int32_t P, float *loss, float *grad_I, double *work,
-                                void *stream) {
-  if (!y || !labels || !loss || !grad_I || !work || B <= 0 || T <= 0 || P <= 0) {
-    set_error("wt_loss_power_ce: invalid argument (null pointer or non-positive size)");
+static bool loss_shape_ok(const char *what, int32_t B, int32_t T, int32_t P) {
+  if (B <= 0 || T <= 0 || P <= 0 || B > 65535) {
+    set_error("%s: invalid shape (B %d, T %d, P %d; 1 <= B <= 65535)", what, B, T, P);
+    return false;
+  }
+  return true;
+}
+
+extern "C" int wt_loss_power_ce_workspace_bytes(int32_t B, int32_t T, int32_t P,
+                                                int64_t *bytes) {
+  if (!bytes) {
+    set_error("wt_loss_power_ce_workspace_bytes: invalid argument (null pointer)");
     return WT_EINVAL;
   }
+  if (!loss_shape_ok("wt_loss_power_ce_workspace_bytes", B, T, P)) return WT_EINVAL;
+  const int64_t C = loss_chunks(B, T);
+  *bytes = ((int64_t)B * C * P + B) * (int64_t)sizeof(double);
+  return WT_OK;
+}
+
+extern "C" int wt_loss_power_ce(const float *y, const int64_t *labels, int32_t B, int32_t T,
+                                int32_t P, float *loss, float *grad_I, void *work,
+                                void *stream) {
+  if (!y || !labels || !loss || !grad_I || !work) {
+    set_error("wt_loss_power_ce: invalid argument (null pointer)");
+    return WT_EINVAL;
+  }
+  if (!loss_shape_ok("wt_loss_power_ce", B, T, P)) return WT_EINVAL;
   if (P > LOSS_MAX_P) {
     set_error("wt_loss_power_ce: %d probes (at most %d)", P, LOSS_MAX_P);
     return WT_EUNSUPPORTED;
   }
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_power_ce_sample, dim3(B), dim3(LOSS_NT), 0, s, y, labels, (int64_t)T, P,
-                     1.0 / (double)B, grad_I, work);
+  const int C = loss_chunks(B, T);
+  double *part = (double *)work;
+  double *loss_b = part + (int64_t)B * C * P;
+  hipLaunchKernelGGL(k_power_ce_partial, dim3(C, B), dim3(LOSS_NT), 0, s, y, (int64_t)T, P,
+                     part);
   WT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_power_ce_mean, dim3(1), dim3(LOSS_NT), 0, s, work, B, loss);
+  hipLaunchKernelGGL(k_power_ce_sample, dim3(B), dim3(LOSS_NT), 0, s, part, C, labels, P,
+                     1.0 / (double)B, grad_I, loss_b);
+  WT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_power_ce_mean, dim3(1), dim3(LOSS_NT), 0, s, loss_b, B, loss);
   WT_HIP(hipGetLastError());
   return WT_OK;
 }

@@ -3,12 +3,16 @@ utils.py:35-36).
 
 ``power_cross_entropy(y, labels)`` equals
 ``torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels)``
--- the reference's criterion on the probe series -- computed by one HIP
-kernel pair (``wt_loss_power_ce``) instead of a chain of torch reductions.
+-- the reference's criterion on the probe series -- computed by three short
+HIP launches (``wt_loss_power_ce``: chunked partial sums over time, the
+per-sample softmax and gradient, the batch mean) instead of a chain of torch
+reductions.
 Its backward hands the time loop dJ/dy as a (B,T,P) view of the (B,P)
 gradient of the integrated powers, expanded over T.
 """
 from __future__ import annotations
+
+import ctypes
 
 import torch
 
@@ -30,8 +34,12 @@ class _PowerCE(torch.autograd.Function):
         lab = labels.to(device=y.device, dtype=torch.int64).contiguous()
         loss = torch.empty((), dtype=torch.float32, device=y.device)
         gI = torch.empty((B, P), dtype=torch.float32, device=y.device)
-        work = torch.empty(B, dtype=torch.float64, device=y.device)
-        _lib.check(_lib.lib().wt_loss_power_ce(
+        L = _lib.lib()
+        nbytes = ctypes.c_int64()
+        _lib.check(L.wt_loss_power_ce_workspace_bytes(B, T, P, ctypes.byref(nbytes)),
+                   "wt_loss_power_ce_workspace_bytes")
+        work = torch.empty((nbytes.value + 7) // 8, dtype=torch.float64, device=y.device)
+        _lib.check(L.wt_loss_power_ce(
             y.data_ptr(), lab.data_ptr(), B, T, P, loss.data_ptr(), gI.data_ptr(),
             work.data_ptr(), _lib.stream_ptr(y.device)), "wt_loss_power_ce")
         ctx.save_for_backward(gI)
