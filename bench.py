@@ -93,6 +93,11 @@ WORKLOADS = {
              "PML, 1000-4000 steps, each with its own rho, a line source and a focal intensity "
              "probe, J = -sum_t u(focal)^2; forward + checkpointed adjoint of every instance as "
              "one captured HIP graph"),
+    "holey_150_h64": dict(
+        kind="geom", N=150, holes=64,
+        metric="Holey geometry c forward+backward time (150x150, 64 holes)",
+        desc="WaveGeometryHoley c = c0 + (c1-c0) P(sum_h exp(-d_h/r_h)) on a 150x150 grid with "
+             "64 holes (beta 100), forward and backward to the hole centres and radii"),
     "scatter_1024": dict(
         kind="forward", bytes=12.0, field_stride=500,
         metric="forward Gcell-steps/s (1024x1024, 4000 steps, snapshots every 500)",
@@ -305,6 +310,72 @@ def cpu_baseline(workload="train_150_b64", budget_s=15.0):
             "sample": f"batch {B}, {Nx}x{Ny}, first {T} of {Tfull} steps, {what}, {el:.1f} s"}
 
 
+def _holey_torch_ops(hx, hy, hr, shape, eta, beta, c0, c1):
+    """geom.py:109-135 as the reference computes it -- a few torch ops per hole
+    -- run on the device: the one-op-per-launch form the kernels replace."""
+    xv = torch.arange(0, shape[0], dtype=hx.dtype, device=hx.device)
+    yv = torch.arange(0, shape[1], dtype=hx.dtype, device=hx.device)
+    x, y = torch.meshgrid(xv, yv, indexing="ij")
+    rho = torch.zeros(shape, dtype=hx.dtype, device=hx.device)
+    for ri, xi, yi in zip(hr, hx, hy):
+        r = torch.sqrt((x - xi).pow(2) + (y - yi).pow(2))
+        rho = rho + torch.exp(-r / ri)
+    p = (np.tanh(beta * eta) + torch.tanh(beta * (rho - eta))) / (
+        np.tanh(beta * eta) + np.tanh(beta * (1 - eta)))
+    return c0 + (c1 - c0) * p
+
+
+def run_geom(args, dev):
+    """Times WaveGeometryHoley.c forward + backward (the wt_geom_holey_c
+    kernels) against the reference's torch ops on the same device."""
+    from wavetorch_amd import WaveGeometryHoley
+    wl = WORKLOADS[args.workload]
+    N, H = wl["N"], wl["holes"]
+    rng = np.random.default_rng(17)
+    hx = (rng.uniform(25, N - 25, H) + 0.3).astype(np.float32)
+    hy = (rng.uniform(25, N - 25, H) + 0.2).astype(np.float32)
+    hr = rng.uniform(2.0, 8.0, H).astype(np.float32)
+    g = WaveGeometryHoley((N, N), 1.0, c0=1.0, c1=0.5, abs_N=20, eta=0.5, beta=100.0, x=hx,
+                          y=hy, r=hr).to(dev)
+    w = torch.from_numpy(rng.standard_normal((N, N)).astype(np.float32)).to(dev)
+    pt = [torch.tensor(v, device=dev, requires_grad=True) for v in (hx, hy, hr)]
+
+    def kern():
+        g.zero_grad(set_to_none=True)
+        (g.c * w).sum().backward()
+
+    def ops():
+        for q in pt:
+            q.grad = None
+        (_holey_torch_ops(*pt, (N, N), 0.5, 100.0, 1.0, 0.5) * w).sum().backward()
+
+    def timed(f, reps):
+        for _ in range(max(args.warmup, 1)):
+            f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    inner = 20
+    dt = timed(kern, args.steps * inner)
+    out = {"metric": wl["metric"], "value": round(dt * 1e6, 1), "unit": "us",
+           "n_gpus": 1, "steps": args.steps * inner, "warmup": args.warmup,
+           "ms_per_step": round(dt * 1e3, 4), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded hole centres and radii)",
+           "config": {"workload": f"{args.workload}: {wl['desc']}", "Nx": N, "Ny": N,
+                      "holes": H}}
+    if not args.no_plain:
+        pdt = timed(ops, max(1, args.steps))
+        log(f"[bench] torch ops: {pdt * 1e6:.1f} us")
+        out["plain_ms_per_step"] = round(pdt * 1e3, 4)
+        out["speedup_vs_plain"] = round(pdt / dt, 3)
+    print(json.dumps(out), flush=True)
+
+
 def run_batch(args, dev):
     """Batched instances: the forward + adjoint of every instance, captured
     once as a HIP graph and replayed; inputs (c of every instance from its
@@ -494,6 +565,9 @@ def main():
     wl = WORKLOADS[args.workload]
     if wl["kind"] == "batch":
         run_batch(args, dev)
+        return
+    if wl["kind"] == "geom":
+        run_geom(args, dev)
         return
     tuning = json.loads(args.tuning) if args.tuning else None
     if tuning is None:

@@ -255,6 +255,29 @@ int wt_geom_freeform_c_bwd(const float *grad_c, const float *blur_in,
                            float c0, float c1, float *grad_rho, float *work,
                            void *stream);
 
+/* c = c0 + (c1-c0) * P(raw) for WaveGeometryHoley (geom.py:109-135):
+ *   raw[i,j] = sum_h exp(-sqrt((i-x_h)^2 + (j-y_h)^2) / r_h), in hole order
+ *   x, y, r  (nholes) hole centres and radii (device; may be NULL if nholes = 0)
+ *   c        (Nx,Ny) output; raw_out (Nx,Ny) receives raw (for the backward)
+ * With c0 = 0, c1 = 1 the output is the projected density (the `rho`
+ * property, geom.py:130-131). */
+int wt_geom_holey_c(const float *x, const float *y, const float *r, int32_t nholes,
+                    int32_t Nx, int32_t Ny, float eta, float beta, float c0,
+                    float c1, float *c, float *raw_out, void *stream);
+
+/* Backward of wt_geom_holey_c: dJ/dx, dJ/dy, dJ/dr (nholes each; any may be
+ * NULL) from grad_c (Nx,Ny) and raw from the forward.  As autograd through
+ * the reference's sqrt, a centre exactly on a grid point gives NaN for its
+ * x and y gradients.  work: wt_geom_holey_workspace_bytes bytes, 8-aligned. */
+int wt_geom_holey_c_bwd(const float *grad_c, const float *raw, const float *x,
+                        const float *y, const float *r, int32_t nholes, int32_t Nx,
+                        int32_t Ny, float eta, float beta, float c0, float c1,
+                        float *grad_x, float *grad_y, float *grad_r, void *work,
+                        void *stream);
+
+/* Bytes of `work` for wt_geom_holey_c_bwd (no counterpart in the reference). */
+int wt_geom_holey_workspace_bytes(int32_t nholes, int32_t Nx, int32_t Ny, int64_t *bytes);
+
 /* ------------------------------------------------------------------------ */
 /* Loss head (train.py:61-62 with utils.py:35-36)                            */
 /* ------------------------------------------------------------------------ */

@@ -20,6 +20,7 @@ allocations:
 * ``pml_b``             geom.py:63-85
 * ``blur_kernel``, ``apply_blur``, ``apply_projection``, ``freeform_c``
                         geom.py:149-154, 207-233
+* ``holey_rho``, ``holey_c``  geom.py:109-135
 * ``normalize_power``   utils.py:35-36
 
 The dtype follows ``torch.get_default_dtype()`` exactly as the reference
@@ -199,6 +200,26 @@ def freeform_c(rho, c0, c1, kernel, blur_N, eta, beta):
     """geom.py:224-233."""
     r = apply_projection(apply_blur(rho, kernel, blur_N), eta, beta)
     return c0 + (c1 - c0) * r
+
+
+def holey_rho(hx, hy, hr, domain_shape, eta, beta):
+    """geom.py:109-128: the projected density of the Holey geometry,
+    P(sum_h exp(-|p - p_h| / r_h)), holes accumulated in order; dtype and
+    device follow the hole tensors."""
+    dt, dev = hx.dtype, hx.device
+    xv = torch.arange(0, domain_shape[0], dtype=dt, device=dev)
+    yv = torch.arange(0, domain_shape[1], dtype=dt, device=dev)
+    x, y = torch.meshgrid(xv, yv, indexing="ij")
+    rho = torch.zeros(domain_shape, dtype=dt, device=dev)
+    for ri, xi, yi in zip(hr, hx, hy):
+        r = torch.sqrt((x - xi).pow(2) + (y - yi).pow(2))
+        rho = rho + torch.exp(-r / ri)
+    return apply_projection(rho, eta, beta)
+
+
+def holey_c(hx, hy, hr, domain_shape, eta, beta, c0, c1):
+    """geom.py:133-135."""
+    return c0 + (c1 - c0) * holey_rho(hx, hy, hr, domain_shape, eta, beta)
 
 
 def normalize_power(X):

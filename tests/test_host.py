@@ -57,6 +57,7 @@ def test_entry_points_reject_null_arguments():
     names = ["wt_coef_linear", "wt_step_fwd", "wt_step_bwd", "wt_plan_resolve",
              "wt_plan_create", "wt_plan_tuning", "wt_workspace_bytes", "wt_loop_fwd",
              "wt_loop_adj", "wt_geom_freeform_c", "wt_geom_freeform_c_bwd",
+             "wt_geom_holey_c", "wt_geom_holey_c_bwd", "wt_geom_holey_workspace_bytes",
              "wt_loss_power_ce", "wt_loss_power_ce_workspace_bytes", "wt_batch_plan_create", "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
              "wt_batch_fwd", "wt_batch_adj", "wt_batch_fwd_adj", "wt_graph_launch"]
     for n in names:
@@ -337,32 +338,46 @@ def test_pml_profile_closed_form():
     assert b[aN, aN] == 0 and abs(b[0, 0] - math.sqrt(2) * sig) < 1e-6
 
 
-def test_holey_geometry_closed_forms():
-    """geom.py:88-133: rho = P(sum_i exp(-|p - p_i| / r_i)) with the tanh
-    projection P (P(0) = 0, P(1) = 1): c = c1 at a lone hole's centre,
-    c -> c0 far from every hole; two coincident holes add before P."""
+def test_holey_oracle_closed_forms():
+    """The oracle's restatement of geom.py:109-135, rho = P(sum_i exp(-|p - p_i| / r_i))
+    with the tanh projection P (P(0) = 0, P(1) = 1): c = c1 at a lone hole's
+    centre, c -> c0 far from every hole; two coincident holes add before P;
+    autograd through it gives NaN x/y gradients for a centre on a grid point,
+    as the reference does, and for off-grid centres agrees in fp64 with
+    central differences of the same function."""
     import numpy as np
-    import torch
-    from wavetorch_amd import WaveGeometryHoley
-    g = WaveGeometryHoley((40, 30), 1.0, c0=1.0, c1=0.5, abs_N=5, abs_sig=1.0, abs_p=2.0,
-                          eta=0.5, beta=100.0, x=[12.0], y=[9.0], r=[1.5])
-    c = g.c.detach()
+    from oracle import ref_torch as rt
+
+    def t(v, dt=torch.float32):
+        return torch.tensor(v, dtype=dt, requires_grad=True)
+    c = rt.holey_c(t([12.0]), t([9.0]), t([1.5]), (40, 30), 0.5, 100.0, 1.0, 0.5).detach()
     assert abs(c[12, 9].item() - 0.5) < 1e-6
     assert abs(c[38, 28].item() - 1.0) < 1e-6
-    # exp(-d/r) at distance 2 from the centre, projected
-    d = np.exp(-2.0 / 1.5)
+    d = np.exp(-2.0 / 1.5)  # exp(-d/r) at distance 2 from the centre, projected
     P = (np.tanh(50.0) + np.tanh(100.0 * (d - 0.5))) / (np.tanh(50.0) + np.tanh(50.0))
-    assert abs(g.rho[14, 9].item() - P) < 1e-5
-    g2 = WaveGeometryHoley((40, 30), 1.0, c0=1.0, c1=0.5, abs_N=5, eta=0.5, beta=4.0,
-                           x=[12.0, 12.0], y=[9.0, 9.0], r=[3.0, 3.0])
-    s = 2 * np.exp(-1.0 / 3.0)
-    P2 = (np.tanh(2.0) + np.tanh(4.0 * (s - 0.5))) / (2 * np.tanh(2.0))
-    assert abs(g2.rho[13, 9].item() - P2) < 1e-5
-    # like the reference, the gradient is NaN for a centre exactly on a grid
-    # point (d sqrt(r^2) at r = 0); off-grid centres give finite gradients
-    g2.c.sum().backward()
-    assert torch.isnan(g2.x.grad).all()
-    g3 = WaveGeometryHoley((40, 30), 1.0, c0=1.0, c1=0.5, abs_N=5, eta=0.5, beta=4.0,
-                           x=[12.3, 20.6], y=[9.4, 17.1], r=[3.0, 2.0])
-    g3.c.sum().backward()
-    assert all(torch.isfinite(t.grad).all() for t in (g3.x, g3.y, g3.r))
+    rho = rt.holey_rho(t([12.0]), t([9.0]), t([1.5]), (40, 30), 0.5, 100.0)
+    assert abs(rho[14, 9].item() - P) < 1e-5
+    hx, hy, hr = t([12.0, 12.0]), t([9.0, 9.0]), t([3.0, 3.0])
+    s2 = 2 * np.exp(-1.0 / 3.0)
+    P2 = (np.tanh(2.0) + np.tanh(4.0 * (s2 - 0.5))) / (2 * np.tanh(2.0))
+    assert abs(rt.holey_rho(hx, hy, hr, (40, 30), 0.5, 4.0)[13, 9].item() - P2) < 1e-5
+    rt.holey_c(hx, hy, hr, (40, 30), 0.5, 4.0, 1.0, 0.5).sum().backward()
+    assert torch.isnan(hx.grad).all() and torch.isnan(hy.grad).all()
+    assert torch.isfinite(hr.grad).all()
+    # off-grid centres, fp64: autograd = central differences
+    p64 = [t([12.3, 20.6], torch.float64), t([9.4, 17.1], torch.float64),
+           t([3.0, 2.0], torch.float64)]
+    w = torch.from_numpy(np.random.default_rng(2).standard_normal((40, 30)))
+
+    def J(a, b, r):
+        return (rt.holey_c(a, b, r, (40, 30), 0.5, 4.0, 1.0, 0.5) * w).sum()
+    J(*p64).backward()
+    eps = 1e-6
+    for k in range(3):
+        for i in range(2):
+            args_p = [q.detach().clone() for q in p64]
+            args_m = [q.detach().clone() for q in p64]
+            args_p[k][i] += eps
+            args_m[k][i] -= eps
+            fd = (J(*args_p) - J(*args_m)).item() / (2 * eps)
+            assert abs(p64[k].grad[i].item() - fd) <= 1e-6 * max(1.0, abs(fd)), (k, i)

@@ -85,6 +85,12 @@ _SIGS = {
                                           c_f32, c_f32, c_p, c_p, c_p, c_p, c_p]),
     "wt_geom_freeform_c_bwd": (ctypes.c_int, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_i32, c_f32,
                                               c_f32, c_f32, c_f32, c_p, c_p, c_p]),
+    "wt_geom_holey_c": (ctypes.c_int, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_f32, c_f32, c_f32,
+                                       c_f32, c_p, c_p, c_p]),
+    "wt_geom_holey_c_bwd": (ctypes.c_int, [c_p, c_p, c_p, c_p, c_p, c_i32, c_i32, c_i32, c_f32,
+                                           c_f32, c_f32, c_f32, c_p, c_p, c_p, c_p, c_p]),
+    "wt_geom_holey_workspace_bytes": (ctypes.c_int, [c_i32, c_i32, c_i32,
+                                                     ctypes.POINTER(c_i64)]),
     "wt_loss_power_ce": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p]),
     "wt_loss_power_ce_workspace_bytes": (ctypes.c_int, [c_i32, c_i32, c_i32,
                                                         ctypes.POINTER(c_i64)]),

@@ -55,6 +55,83 @@ __global__ void k_proj_c_bwd(const float *__restrict__ gc, const float *__restri
   gx[q] = gc[q] * dc * beta * (1.0f - t * t) / D;
 }
 
+// ---------------------------------------------------------------- Holey
+//
+// WaveGeometryHoley (geom.py:109-135): the density before projection is
+//   raw[i,j] = sum_h exp(-d_h / r_h),  d_h = sqrt((i - x_h)^2 + (j - y_h)^2)
+// accumulated in fp32 in hole order with the reference's operation order
+// ((i - x)^2 + (j - y)^2, sqrt, (-d) / r, exp).  The projection and c reuse
+// k_proj_c / k_proj_c_bwd.  Hole parameters are read with uniform loads.
+__global__ void k_holey_raw(const float *__restrict__ hx, const float *__restrict__ hy,
+                            const float *__restrict__ hr, int H, int Nx, int Ny,
+                            float *__restrict__ raw) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Nx * Ny) return;
+  const float fi = (float)(q / Ny), fj = (float)(q - (q / Ny) * Ny);
+  float s = 0.0f;
+  for (int h = 0; h < H; ++h) {
+    const float dx = fi - hx[h], dy = fj - hy[h];
+    const float d = sqrtf(dx * dx + dy * dy);
+    s = s + expf(-d / hr[h]);
+  }
+  raw[q] = s;
+}
+
+// Hole gradients from g = dJ/d(raw): with e = exp(-d/r),
+//   dJ/dx_h = sum g e (i - x_h) / (r_h d),  dJ/dy_h = sum g e (j - y_h) / (r_h d),
+//   dJ/dr_h = sum g e d / r_h^2
+// (autograd through sqrt gives the same 0/0 = NaN at a centre that sits on
+// a grid point).  Block (c, h) sums cells [c*per, (c+1)*per) for hole h in
+// double; k_holey_grad_final adds the C partials.
+constexpr int HOLEY_NT = 256;
+constexpr int HOLEY_CELLS = 2048;  // cells per partial-sum workgroup
+
+__global__ __launch_bounds__(HOLEY_NT) void k_holey_grad_partial(
+    const float *__restrict__ g, const float *__restrict__ hx, const float *__restrict__ hy,
+    const float *__restrict__ hr, int Nx, int Ny, double *__restrict__ part) {
+  __shared__ double red[3][HOLEY_NT];
+  const int c = blockIdx.x, C = gridDim.x, h = blockIdx.y, tid = threadIdx.x;
+  const int n = Nx * Ny;
+  const int q0 = c * HOLEY_CELLS, q1 = min(n, q0 + HOLEY_CELLS);
+  const float xh = hx[h], yh = hy[h], rh = hr[h];
+  double ax = 0.0, ay = 0.0, ar = 0.0;
+  for (int q = q0 + tid; q < q1; q += HOLEY_NT) {
+    const float dx = (float)(q / Ny) - xh, dy = (float)(q - (q / Ny) * Ny) - yh;
+    const float d = sqrtf(dx * dx + dy * dy);
+    const float e = expf(-d / rh);
+    const double ge = (double)g[q] * (double)e;
+    ax += ge * ((double)dx / ((double)rh * (double)d));
+    ay += ge * ((double)dy / ((double)rh * (double)d));
+    ar += ge * ((double)d / ((double)rh * (double)rh));
+  }
+  red[0][tid] = ax; red[1][tid] = ay; red[2][tid] = ar;
+  __syncthreads();
+  for (int w = HOLEY_NT / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      red[0][tid] += red[0][tid + w];
+      red[1][tid] += red[1][tid + w];
+      red[2][tid] += red[2][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid < 3) part[((int64_t)h * C + c) * 3 + tid] = red[tid][0];
+}
+
+__global__ void k_holey_grad_final(const double *__restrict__ part, int H, int C,
+                                   float *__restrict__ gx, float *__restrict__ gy,
+                                   float *__restrict__ gr) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= H) return;
+  double sx = 0.0, sy = 0.0, sr = 0.0;
+  for (int c = 0; c < C; ++c) {
+    const double *p = part + ((int64_t)h * C + c) * 3;
+    sx += p[0]; sy += p[1]; sr += p[2];
+  }
+  if (gx) gx[h] = (float)sx;
+  if (gy) gy[h] = (float)sy;
+  if (gr) gr[h] = (float)sr;
+}
+
 __global__ void k_mul_scalar(const float *__restrict__ in, float v, int64_t n,
                              float *__restrict__ out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -154,6 +231,86 @@ extern "C" int wt_geom_freeform_c_bwd(const float *grad_c, const float *blur_in,
                        dst);
     cur = dst;
   }
+  WT_HIP(hipGetLastError());
+  return WT_OK;
+}
+
+// ---------------------------------------------------------------- Holey entry points
+
+static int holey_chunks(int Nx, int Ny) {
+  return (int)(((int64_t)Nx * Ny + HOLEY_CELLS - 1) / HOLEY_CELLS);
+}
+
+static bool holey_args_ok(const char *what, int32_t H, int32_t Nx, int32_t Ny) {
+  if (H < 0 || H > 65535 || Nx <= 0 || Ny <= 0 || (int64_t)Nx * Ny > (int64_t)1 << 30) {
+    set_error("%s: invalid shape (holes %d, %d x %d)", what, H, Nx, Ny);
+    return false;
+  }
+  return true;
+}
+
+extern "C" int wt_geom_holey_workspace_bytes(int32_t nholes, int32_t Nx, int32_t Ny,
+                                             int64_t *bytes) {
+  if (!bytes) {
+    set_error("wt_geom_holey_workspace_bytes: invalid argument (null pointer)");
+    return WT_EINVAL;
+  }
+  if (!holey_args_ok("wt_geom_holey_workspace_bytes", nholes, Nx, Ny)) return WT_EINVAL;
+  const int64_t n = (int64_t)Nx * Ny;
+  // dJ/d(raw) (n floats, padded to 8 bytes), then the partial sums
+  *bytes = (n * 4 + 7) / 8 * 8 + (int64_t)nholes * holey_chunks(Nx, Ny) * 3 * 8;
+  return WT_OK;
+}
+
+extern "C" int wt_geom_holey_c(const float *hx, const float *hy, const float *hr,
+                               int32_t nholes, int32_t Nx, int32_t Ny, float eta, float beta,
+                               float c0, float c1, float *c, float *raw_out, void *stream) {
+  if (!c || !raw_out || (nholes > 0 && (!hx || !hy || !hr))) {
+    set_error("wt_geom_holey_c: invalid argument (null pointer)");
+    return WT_EINVAL;
+  }
+  if (!holey_args_ok("wt_geom_holey_c", nholes, Nx, Ny)) return WT_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  const int n = Nx * Ny;
+  hipLaunchKernelGGL(k_holey_raw, dim3(nb(n)), dim3(256), 0, s, hx, hy, hr, nholes, Nx, Ny,
+                     raw_out);
+  WT_HIP(hipGetLastError());
+  float T1, D;
+  proj_consts(eta, beta, T1, D);
+  const float dc = (float)((double)c1 - (double)c0);
+  hipLaunchKernelGGL(k_proj_c, dim3(nb(n)), dim3(256), 0, s, raw_out, n, T1, D, beta, eta, c0,
+                     dc, c, (float *)nullptr);
+  WT_HIP(hipGetLastError());
+  return WT_OK;
+}
+
+extern "C" int wt_geom_holey_c_bwd(const float *grad_c, const float *raw, const float *hx,
+                                   const float *hy, const float *hr, int32_t nholes, int32_t Nx,
+                                   int32_t Ny, float eta, float beta, float c0, float c1,
+                                   float *grad_x, float *grad_y, float *grad_r, void *work,
+                                   void *stream) {
+  if (!grad_c || !raw || !work || (nholes > 0 && (!hx || !hy || !hr))) {
+    set_error("wt_geom_holey_c_bwd: invalid argument (null pointer)");
+    return WT_EINVAL;
+  }
+  if (!holey_args_ok("wt_geom_holey_c_bwd", nholes, Nx, Ny)) return WT_EINVAL;
+  if (nholes == 0) return WT_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int n = Nx * Ny;
+  float T1, D;
+  proj_consts(eta, beta, T1, D);
+  const float dc = (float)((double)c1 - (double)c0);
+  float *g = (float *)work;
+  double *part = (double *)((char *)work + ((int64_t)n * 4 + 7) / 8 * 8);
+  hipLaunchKernelGGL(k_proj_c_bwd, dim3(nb(n)), dim3(256), 0, s, grad_c, raw, n, D, beta, eta,
+                     dc, g);
+  WT_HIP(hipGetLastError());
+  const int C = holey_chunks(Nx, Ny);
+  hipLaunchKernelGGL(k_holey_grad_partial, dim3(C, nholes), dim3(HOLEY_NT), 0, s, g, hx, hy, hr,
+                     Nx, Ny, part);
+  WT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_holey_grad_final, dim3((nholes + 255) / 256), dim3(256), 0, s, part,
+                     nholes, C, grad_x, grad_y, grad_r);
   WT_HIP(hipGetLastError());
   return WT_OK;
 }

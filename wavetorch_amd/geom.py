@@ -5,7 +5,9 @@ reference.  The PML damping field ``b`` is built once at construction
 (geom.py:63-85).  ``WaveGeometryFreeForm.c`` -- blur, tanh projection and
 the linear map to wave speed (geom.py:207-233) -- runs as gfx950 kernels
 (``wt_geom_freeform_c``) with a hand-written backward, so autograd carries
-dJ/dc from the time loop back to ``rho``.
+dJ/dc from the time loop back to ``rho``; ``WaveGeometryHoley.c`` -- the sum
+of exponential holes and the same projection (geom.py:109-135) -- likewise
+(``wt_geom_holey_c``), back to the hole centres and radii.
 """
 from __future__ import annotations
 
@@ -112,19 +114,29 @@ class WaveGeometryHoley(WaveGeometry):
                    "r": deepcopy(self.r.detach())}
         return {**super().state_reconstruction_args(), **my_args}
 
+    def _host_params(self):
+        """(eta, beta, c0, c1) as host numbers, re-read from the device buffers
+        only when one of them was replaced or modified (data_ptr, _version):
+        reading them on every call would synchronise the host with the device
+        once per training step."""
+        ts = (self.eta, self.beta, self.c0, self.c1)
+        key = tuple((t.data_ptr(), t._version) for t in ts)
+        cached = getattr(self, "_host_cache", None)
+        if cached is None or cached[0] != key:
+            cached = (key, tuple(float(t.item()) for t in ts))
+            self._host_cache = cached
+        return cached[1]
+
+    def _holes(self, c0, c1):
+        _require_hip(self.x, "WaveGeometryHoley")
+        eta, beta, _, _ = self._host_params()
+        Nx, Ny = self.domain_shape
+        return _HoleyC.apply(self.x, self.y, self.r, Nx, Ny, eta, beta, c0, c1)
+
     def _rho(self):
-        eta = self.eta.item()
-        beta = self.beta.item()
-        dev = self.x.device
-        xv = torch.arange(0, self.domain_shape[0], dtype=torch.get_default_dtype(), device=dev)
-        yv = torch.arange(0, self.domain_shape[1], dtype=torch.get_default_dtype(), device=dev)
-        x, y = torch.meshgrid(xv, yv, indexing="ij")
-        rho = torch.zeros(self.domain_shape, device=dev)
-        for ri, xi, yi in zip(self.r, self.x, self.y):
-            r = torch.sqrt((x - xi).pow(2) + (y - yi).pow(2))
-            rho = rho + torch.exp(-r / ri)
-        return (np.tanh(beta * eta) + torch.tanh(beta * (rho - eta))) / (
-            np.tanh(beta * eta) + np.tanh(beta * (1 - eta)))
+        """Projected density P(sum_h exp(-d_h / r_h)) (geom.py:109-128), gfx950
+        kernels (the same kernels as ``c`` with c0 = 0, c1 = 1)."""
+        return self._holes(0.0, 1.0)
 
     @property
     def rho(self):
@@ -132,7 +144,53 @@ class WaveGeometryHoley(WaveGeometry):
 
     @property
     def c(self):
-        return self.c0.item() + (self.c1.item() - self.c0.item()) * self._rho()
+        """c0 + (c1 - c0) * rho (geom.py:133-135), gfx950 kernels."""
+        _, _, c0, c1 = self._host_params()
+        return self._holes(c0, c1)
+
+
+class _HoleyC(torch.autograd.Function):
+    """c = c0 + (c1-c0) * P(sum_h exp(-d_h / r_h)) on the GPU (geom.py:109-135),
+    with the hand-written backward for the hole centres and radii."""
+
+    @staticmethod
+    def forward(ctx, hx, hy, hr, Nx, Ny, eta, beta, c0, c1):
+        shapes = (hx.shape, hy.shape, hr.shape)
+        hx, hy, hr = (t.reshape(-1).contiguous() for t in (hx, hy, hr))
+        H = hx.numel()
+        if hy.numel() != H or hr.numel() != H:
+            raise ValueError("WaveGeometryHoley: x, y and r must have one entry per hole")
+        for t in (hy, hr):
+            _require_hip(t, "WaveGeometryHoley")
+        c = torch.empty((Nx, Ny), dtype=torch.float32, device=hx.device)
+        raw = torch.empty_like(c)
+        _lib.check(_lib.lib().wt_geom_holey_c(
+            hx.data_ptr(), hy.data_ptr(), hr.data_ptr(), H, Nx, Ny, eta, beta, c0, c1,
+            c.data_ptr(), raw.data_ptr(), _lib.stream_ptr(hx.device)), "wt_geom_holey_c")
+        ctx.save_for_backward(raw, hx, hy, hr)
+        ctx.params = (Nx, Ny, eta, beta, c0, c1)
+        ctx.shapes = shapes
+        return c
+
+    @staticmethod
+    def backward(ctx, grad_c):
+        import ctypes
+        raw, hx, hy, hr = ctx.saved_tensors
+        Nx, Ny, eta, beta, c0, c1 = ctx.params
+        H = hx.numel()
+        need = ctx.needs_input_grad[:3]
+        out = [torch.empty_like(t) if n else None for t, n in zip((hx, hy, hr), need)]
+        L = _lib.lib()
+        nbytes = ctypes.c_int64()
+        _lib.check(L.wt_geom_holey_workspace_bytes(H, Nx, Ny, ctypes.byref(nbytes)),
+                   "wt_geom_holey_workspace_bytes")
+        work = torch.empty((nbytes.value + 7) // 8, dtype=torch.float64, device=raw.device)
+        _lib.check(L.wt_geom_holey_c_bwd(
+            grad_c.contiguous().data_ptr(), raw.data_ptr(), hx.data_ptr(), hy.data_ptr(),
+            hr.data_ptr(), H, Nx, Ny, eta, beta, c0, c1, *(_lib.ptr(o) for o in out),
+            work.data_ptr(), _lib.stream_ptr(raw.device)), "wt_geom_holey_c_bwd")
+        grads = [o.reshape(s) if o is not None else None for o, s in zip(out, ctx.shapes)]
+        return (*grads, None, None, None, None, None, None)
 
 
 def _disk_pixels(radius: int) -> np.ndarray:
