@@ -8,7 +8,8 @@ train.py:13-133, utils.py:35-36).
   kernel), over batch/probe/length edge cases, with the oracle's numpy
   restatement of the same loss as a second check.
 * GraphedTrainStep: three replays with new batches, each equal bit for bit
-  to the eager step (same kernels in the same order) run between them.
+  to the eager step (same kernels in the same order) run between them, for
+  the linear and the nonlinear cell.
 * train(): the reference's epoch loop and history layout on a small model,
   epoch 0 without an optimizer step, the fused criterion's epoch-0 loss equal
   to the torch composition, and a saved file that loads back.
@@ -61,11 +62,13 @@ def test_power_cross_entropy_rejects_and_flags(cuda):
         power_cross_entropy(torch.rand((2, 5), device=cuda), torch.zeros(2, device=cuda))
 
 
-def _model(cuda, seed):
+def _model(cuda, seed, nonlinear=False):
     torch.manual_seed(seed)
     geom = WaveGeometryFreeForm((60, 50), 1.0, c0=1.0, c1=0.5, abs_N=10, abs_sig=1.0,
                                 abs_p=2.0, rho="rand")
-    m = WaveRNN(WaveCell(0.5, geom), [WaveSource(20, 25)],
+    nl = dict(satdamp_b0=0.5, satdamp_uth=0.3, c_nl=0.05, c_clamp=(0.4, 1.1)) if nonlinear \
+        else {}
+    m = WaveRNN(WaveCell(0.5, geom, **nl), [WaveSource(20, 25)],
                 [WaveIntensityProbe(40, 10), WaveIntensityProbe(40, 25),
                  WaveIntensityProbe(40, 40)])
     return m.to(cuda)
@@ -80,9 +83,10 @@ def _eager(m, opt, x, labels):
     return loss
 
 
-def test_graphed_train_step_equals_eager(cuda):
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_graphed_train_step_equals_eager(cuda, nonlinear):
     B, T = 4, 200
-    ma, mb = _model(cuda, 7), _model(cuda, 7)
+    ma, mb = _model(cuda, 7, nonlinear), _model(cuda, 7, nonlinear)
     oa = torch.optim.Adam(ma.parameters(), lr=1e-2, capturable=True)
     ob = torch.optim.Adam(mb.parameters(), lr=1e-2, capturable=True)
     g = torch.Generator().manual_seed(3)
