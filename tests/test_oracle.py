@@ -364,3 +364,19 @@ def test_nonlinear_training_gradient_conditioning():
         g1 = co.adjoint(b, c1, xn[:, :T], cfg.dt, 1.0, src, taps, ybar=w[:, :T], nl=nl)
         r = rel(g1["gc"], g0["gc"])
         assert lo <= r < hi, (T, r)
+
+
+def test_ce_loss_matches_torch_composition():
+    """The numpy loss head of the parity workers (tests/parity_cpu.py: ce_loss)
+    equals torch's cross_entropy(normalize_power(y.sum(dim=1)), labels)
+    (train.py:61-62) in float64."""
+    import torch
+    from tests.parity_cpu import ce_loss
+    from wavetorch_amd.utils import normalize_power
+    rng = np.random.default_rng(4)
+    for B, T, P in ((6, 50, 3), (1, 7, 5), (9, 3, 2)):
+        y = rng.uniform(1e-4, 0.02, (B, T, P))
+        lab = rng.integers(0, P, B)
+        want = torch.nn.functional.cross_entropy(
+            normalize_power(torch.from_numpy(y).sum(dim=1)), torch.from_numpy(lab)).item()
+        assert abs(ce_loss({"labels": lab}, y) - want) <= 1e-14 * max(1.0, abs(want))
