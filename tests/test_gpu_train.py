@@ -9,7 +9,7 @@ train.py:13-133, utils.py:35-36).
   restatement of the same loss as a second check.
 * GraphedTrainStep: three replays with new batches, each equal bit for bit
   to the eager step (same kernels in the same order) run between them, for
-  the linear and the nonlinear cell.
+  the linear and the nonlinear cell and the Holey geometry.
 * train(): the reference's epoch loop and history layout on a small model,
   epoch 0 without an optimizer step, the fused criterion's epoch-0 loss equal
   to the torch composition, and a saved file that loads back.
@@ -21,7 +21,7 @@ import torch
 from tests.helpers import rel
 from tests.parity_cpu import ce_loss
 from wavetorch_amd import (GraphedTrainStep, WaveCell, WaveGeometryFreeForm,
-                           WaveIntensityProbe, WaveRNN, WaveSource, train)
+                           WaveGeometryHoley, WaveIntensityProbe, WaveRNN, WaveSource, train)
 from wavetorch_amd.loss import power_cross_entropy
 from wavetorch_amd.utils import accuracy_onehot, normalize_power
 
@@ -62,10 +62,15 @@ def test_power_cross_entropy_rejects_and_flags(cuda):
         power_cross_entropy(torch.rand((2, 5), device=cuda), torch.zeros(2, device=cuda))
 
 
-def _model(cuda, seed, nonlinear=False):
+def _model(cuda, seed, nonlinear=False, holey=False):
     torch.manual_seed(seed)
-    geom = WaveGeometryFreeForm((60, 50), 1.0, c0=1.0, c1=0.5, abs_N=10, abs_sig=1.0,
-                                abs_p=2.0, rho="rand")
+    if holey:
+        geom = WaveGeometryHoley((60, 50), 1.0, c0=1.0, c1=0.5, abs_N=10, abs_sig=1.0,
+                                 abs_p=2.0, eta=0.5, beta=4.0, x=[25.3, 38.6], y=[17.4, 31.1],
+                                 r=[3.0, 2.5])
+    else:
+        geom = WaveGeometryFreeForm((60, 50), 1.0, c0=1.0, c1=0.5, abs_N=10, abs_sig=1.0,
+                                    abs_p=2.0, rho="rand")
     nl = dict(satdamp_b0=0.5, satdamp_uth=0.3, c_nl=0.05, c_clamp=(0.4, 1.1)) if nonlinear \
         else {}
     m = WaveRNN(WaveCell(0.5, geom, **nl), [WaveSource(20, 25)],
@@ -83,19 +88,21 @@ def _eager(m, opt, x, labels):
     return loss
 
 
-@pytest.mark.parametrize("nonlinear", [False, True])
-def test_graphed_train_step_equals_eager(cuda, nonlinear):
+@pytest.mark.parametrize("nonlinear,holey", [(False, False), (True, False), (False, True)])
+def test_graphed_train_step_equals_eager(cuda, nonlinear, holey):
     B, T = 4, 200
-    ma, mb = _model(cuda, 7, nonlinear), _model(cuda, 7, nonlinear)
+    ma, mb = _model(cuda, 7, nonlinear, holey), _model(cuda, 7, nonlinear, holey)
+    pa = next(iter(ma.parameters()))
+    pb = next(iter(mb.parameters()))
     oa = torch.optim.Adam(ma.parameters(), lr=1e-2, capturable=True)
     ob = torch.optim.Adam(mb.parameters(), lr=1e-2, capturable=True)
     g = torch.Generator().manual_seed(3)
     x0 = torch.randn((B, T), generator=g).to(cuda)
     l0 = torch.randint(0, 3, (B,), generator=g).to(cuda)
-    rho0 = mb.cell.geom.rho.detach().clone()
+    p0 = pb.detach().clone()
     step = GraphedTrainStep(mb, ob, x0, l0, warmup=2)
     # building the step leaves the model as it was
-    assert torch.equal(mb.cell.geom.rho.detach(), rho0)
+    assert torch.equal(pb.detach(), p0)
     for i in range(3):
         x = torch.randn((B, T), generator=g).to(cuda)
         labels = torch.randint(0, 3, (B,), generator=g).to(cuda)
@@ -103,11 +110,12 @@ def test_graphed_train_step_equals_eager(cuda, nonlinear):
         lb = step(x, labels)
         torch.cuda.synchronize()
         assert la.item() == lb.item(), (i, la.item(), lb.item())
-        assert torch.equal(ma.cell.geom.rho.detach(), mb.cell.geom.rho.detach()), i
-        sa, sb = oa.state[ma.cell.geom.rho], ob.state[mb.cell.geom.rho]
+        for qa, qb in zip(ma.parameters(), mb.parameters()):
+            assert torch.equal(qa.detach(), qb.detach()), i
+        sa, sb = oa.state[pa], ob.state[pb]
         assert torch.equal(sa["exp_avg"], sb["exp_avg"]) and \
             torch.equal(sa["exp_avg_sq"], sb["exp_avg_sq"]), i
-    assert not torch.equal(mb.cell.geom.rho.detach(), rho0)
+    assert not torch.equal(pb.detach(), p0)
     with pytest.raises(ValueError, match="captured for"):
         step(torch.zeros((B + 1, T), device=cuda), torch.zeros(B + 1, device=cuda))
 
