@@ -139,6 +139,19 @@ def test_autotune_tiling_options():
     assert autotune._from_tuning(tun) == st
 
 
+def test_batch_candidates_checkpoint_interval():
+    """Batch tuning candidates carry K = 64 and the smallest K >= 64 that both
+    block depths divide (segments then end on full launches)."""
+    from wavetorch_amd import autotune
+    assert autotune._aligned_k(24, 24) == 72 and autotune._aligned_k(16, 16) == 64
+    assert autotune._aligned_k(32, 24) == 96 and autotune._aligned_k(24, 16) == 96
+    for c in autotune.BATCH_CANDIDATES:
+        k = c["ck_every"]
+        assert k == 64 or (k % c["tblock"] == 0 and k % c["a_tblock"] == 0 and k <= 256)
+    assert {(c["tblock"], c["a_tblock"], c["ck_every"]) for c in autotune.BATCH_CANDIDATES} \
+        >= {(24, 24, 64), (24, 24, 72)}
+
+
 def test_resolve_single_tile_runs_whole_segments():
     """One tile per sample has no halo: one launch per checkpoint segment."""
     cfg = LoopConfig(Nx=40, Ny=36, dt=0.707, h=1.0, src_px=((9, 18, 0),), taps=((30, 12, 1),))

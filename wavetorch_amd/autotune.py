@@ -14,6 +14,7 @@ atomically; a corrupt file is ignored and rewritten).
 from __future__ import annotations
 
 import json
+import math
 import os
 import threading
 import time
@@ -219,10 +220,23 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
 
 # ------------------------------------------------------------ batched instances
 
+def _aligned_k(tb, atb, kmin=64):
+    """The smallest checkpoint interval >= kmin that both block depths
+    divide: no segment then ends in a shorter launch."""
+    step = tb * atb // math.gcd(tb, atb)
+    return -(-kmin // step) * step
+
+
+# Batched instances: block variant and depth of forward and adjoint, and the
+# checkpoint interval (64, or the smallest multiple of both depths from 64 up;
+# the lens batch's configuration leaves K open).
 BATCH_CANDIDATES = [
-    {"tblock": tb, "nt": 1024, "cpt": r, "a_tblock": atb, "a_nt": 1024, "a_cpt": ar}
+    {"tblock": tb, "nt": 1024, "cpt": r, "a_tblock": atb, "a_nt": 1024, "a_cpt": ar,
+     "ck_every": k}
     for tb, r in ((16, 10), (24, 10), (24, 12), (32, 12))
     for atb, ar in ((13, 8), (16, 8), (16, 10), (20, 10), (24, 10))
+    for k in sorted({64, _aligned_k(tb, atb)})
+    if k <= 256
 ]
 
 
@@ -242,9 +256,9 @@ def cached_batch(insts, arch=None):
 
 def tune_batch(insts, cs, xs, ybars, device, candidates=None, reps=2, verbose=False):
     """Times the forward + adjoint of the whole batch (wt_batch_fwd_adj) for
-    each candidate block variant / time-block depth (the per-group tiling
-    follows from each: the largest square tiles that fit) and caches the
-    fastest under batch_key(insts)."""
+    each candidate block variant / time-block depth / checkpoint interval (the
+    per-group tiling follows from each: the largest square tiles that fit) and
+    caches the fastest under batch_key(insts)."""
     from .batch import InstanceBatch
     best, best_t = None, float("inf")
     for cand in candidates or BATCH_CANDIDATES:
