@@ -93,9 +93,13 @@ def test_resolve_training_shape_heuristic():
         (2, 2, 24, 1024, 10)
     assert t["ck_every"] == 64
     F = 150 * 150
-    # checkpoints (u_n, u_{n-1}) for ceil(T/K) segments dominate the saved bytes
-    assert saved >= math.ceil(3000 / 64) * 2 * 64 * F * 4
-    assert saved < math.ceil(3000 / 64) * 2 * 64 * F * 4 + 8 * 2 ** 20
+    # checkpoints (u_n, u_{n-1}) for ceil(T/K) segments and the last segment's
+    # record (lap u of K steps, rows padded to Ny + 1), which the forward
+    # writes, dominate the saved bytes
+    ck = math.ceil(3000 / 64) * 2 * 64 * F * 4
+    rec = 64 * 64 * 150 * 151 * 4
+    assert saved >= ck + rec
+    assert saved < ck + rec + 8 * 2 ** 20
 
 
 def nl_cfg():
@@ -137,6 +141,19 @@ def test_autotune_tiling_options():
     tun = autotune._to_tuning(st)
     assert (tun["strips"], tun["tcols"], tun["a_strips"], tun["a_tcols"]) == (4, 2, 2, 4)
     assert autotune._from_tuning(tun) == st
+
+
+def test_tuner_checkpoint_interval_options():
+    """With tune_k the single-plan search tries K0, its doublings up to T, and
+    T (one segment, no recompute); without it the tuning carries no K."""
+    from wavetorch_amd import autotune
+    assert autotune.k_options(64, 1000) == (64, 128, 256, 512, 1000)
+    assert autotune.k_options(64, 64) == (64,)
+    st = {"tiling": (6, 5), "tblock": 16, "cpt": 4, "a_tiling": (5, 6), "a_tblock": 16,
+          "a_cpt": 4}
+    assert "ck_every" not in autotune._to_tuning(st)
+    t = autotune._to_tuning(dict(st, ck_every=1000))
+    assert t["ck_every"] == 1000 and autotune._from_tuning(t, with_k=True)["ck_every"] == 1000
 
 
 def test_batch_candidates_checkpoint_interval():

@@ -103,20 +103,33 @@ def tilings(Nx, Ny, s0, c0):
 
 
 def _to_tuning(st):
-    return {"kind": 0, "strips": st["tiling"][0], "tcols": st["tiling"][1],
-            "tblock": st["tblock"], "nt": 1024, "cpt": st["cpt"],
-            "a_strips": st["a_tiling"][0], "a_tcols": st["a_tiling"][1],
-            "a_tblock": st["a_tblock"], "a_nt": 1024, "a_cpt": st["a_cpt"]}
+    t = {"kind": 0, "strips": st["tiling"][0], "tcols": st["tiling"][1],
+         "tblock": st["tblock"], "nt": 1024, "cpt": st["cpt"],
+         "a_strips": st["a_tiling"][0], "a_tcols": st["a_tiling"][1],
+         "a_tblock": st["a_tblock"], "a_nt": 1024, "a_cpt": st["a_cpt"]}
+    if "ck_every" in st:
+        t["ck_every"] = st["ck_every"]
+    return t
 
 
-def _from_tuning(t):
-    return {"tiling": (t["strips"], t["tcols"]), "tblock": t["tblock"], "cpt": t["cpt"],
-            "a_tiling": (t["a_strips"], t["a_tcols"]), "a_tblock": t["a_tblock"],
-            "a_cpt": t["a_cpt"]}
+def _from_tuning(t, with_k=False):
+    st = {"tiling": (t["strips"], t["tcols"]), "tblock": t["tblock"], "cpt": t["cpt"],
+          "a_tiling": (t["a_strips"], t["a_tcols"]), "a_tblock": t["a_tblock"],
+          "a_cpt": t["a_cpt"]}
+    if with_k:
+        st["ck_every"] = t["ck_every"]
+    return st
+
+
+def k_options(K0, T):
+    """Checkpoint intervals the search tries from the model's K0: doublings
+    up to T, and T itself (one segment: the forward then writes the only
+    record and the reverse sweep recomputes nothing)."""
+    return tuple(sorted({k for k in (K0, 2 * K0, 4 * K0, 8 * K0, T) if K0 <= k <= T}))
 
 
 def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2,
-         forward_only=False, reps=2):
+         forward_only=False, reps=2, tune_k=False):
     """Coordinate search over the tuning fields of AXES for ``model`` on
     batch ``x``, timing a full training step (forward, loss, checkpointed
     adjoint) on the GPU, or with `forward_only` the forward without gradients
@@ -126,7 +139,10 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
     between good plans).  Starts from the library's own choice, tries every
     option of one field at a time (the rest fixed), keeps any improvement and
     repeats for `rounds` passes or until `budget_s`; caches the winner under
-    key(Nx, Ny, B, T, K) and returns it."""
+    key(Nx, Ny, B, T, K) and returns it.  With `tune_k` the checkpoint
+    interval is searched too (k_options; the winner carries ``ck_every``) --
+    for configurations that leave K open: more memory for fewer recomputed
+    segments."""
     from .utils import normalize_power
     B, T = x.shape
     Nx, Ny = model.cell.geom.domain_shape
@@ -171,7 +187,7 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
     base, best_t = measure({})
     if base is None:
         return None
-    state = _from_tuning(base)
+    state = _from_tuning(base, with_k=tune_k)
     tried = {tuple(sorted(state.items()))}
     def consider(cand):
         nonlocal state, best_t
@@ -192,6 +208,8 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
         axes[f] = tilings(Nx, Ny, *state[f])
     if forward_only:  # the adjoint's fields do not take part
         axes = {f: o for f, o in axes.items() if not f.startswith("a_")}
+    if tune_k and not forward_only:
+        axes["ck_every"] = k_options(K, T)
     for _ in range(rounds):
         improved = False
         for field, options in axes.items():

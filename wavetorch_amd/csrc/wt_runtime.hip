@@ -164,7 +164,7 @@ struct Plan {
   bool c2ps = false;
   // workspace layout (bytes)
   size_t F = 0;
-  size_t saved_coef = 0, saved_ck = 0, saved_yraw = 0, saved_bytes = 0;
+  size_t saved_coef = 0, saved_ck = 0, saved_yraw = 0, saved_rec = 0, saved_bytes = 0;
   size_t t_state = 0, t_lam = 0, t_wl = 0, t_gacc = 0, t_seg = 0, t_gsrc = 0, t_extra = 0,
          temp_bytes = 0;
   size_t temp_bytes_nograd = 0;
@@ -367,7 +367,12 @@ static int plan_layout(Plan &p) {
   p.saved_coef = align_up(ncoef * F * sizeof(float));
   p.saved_ck = align_up((size_t)p.nseg * 2 * BF);
   p.saved_yraw = align_up((size_t)p.B * p.T * std::max(p.ntap, 1) * sizeof(float));
-  p.saved_bytes = p.saved_coef + p.saved_ck + p.saved_yraw;
+  // the record of the last segment, written by the forward itself: the
+  // reverse sweep starts there and needs no recompute for it (one segment in
+  // all when K >= T).  Batched plans keep their records in temp only.
+  const size_t segcells = (size_t)p.B * p.K * p.Nx * (p.Ny + 1);
+  p.saved_rec = p.c2ps ? 0 : align_up((nlc ? 2 : 1) * segcells * sizeof(float));
+  p.saved_bytes = p.saved_coef + p.saved_ck + p.saved_yraw + p.saved_rec;
   // temp
   p.t_state = align_up(8 * BF);  // 4 state pairs (ping-pong + segment in/out)
   // adjoint state ping-pong (plain uses 3); the nonlinear blocked adjoint
@@ -375,7 +380,6 @@ static int plan_layout(Plan &p) {
   p.t_lam = align_up((nlc ? 12 : 4) * BF);
   p.t_wl = align_up(2 * BF);     // plain adjoint w*lambda
   p.t_gacc = align_up((nlc ? 3 : 1) * BF);
-  size_t segcells = (size_t)p.B * p.K * p.Nx * (p.Ny + 1);
   p.t_seg = align_up((nlc ? 2 : 1) * segcells * sizeof(float));
   p.t_gsrc = align_up((size_t)p.B * p.T * std::max(p.nslot, 1) * sizeof(float));
   p.t_extra = align_up(ncoef * F * sizeof(float));  // coefficients when not saving
@@ -892,6 +896,7 @@ struct Ws {
   float *wl;                  // 2 fields
   float *gacc;                // 1 (linear) or 3 fields
   float *seg;                 // segment record
+  float *rec;                 // last segment's record from the forward (or null)
   float *gsrc;
 };
 
@@ -908,6 +913,8 @@ Ws layout(const Plan &p, void *saved, void *temp, bool with_grad) {
     w.ck = (float *)s;
     s += p.saved_ck;
     w.yraw = (float *)s;
+    s += p.saved_yraw;
+    w.rec = p.saved_rec ? (float *)s : nullptr;
     w.lam = (float *)t;
     t += p.t_lam;
     w.wl = (float *)t;
@@ -1072,8 +1079,14 @@ int adj_nonlinear(const Plan &p, const float *b, const float *c, const float *rh
       const float *ck = w.ck + (size_t)sg * 2 * BF;
       fa.rec_n0 = n0;
       float *tmp[4] = {st[0], st[1], st[2], st[3]};
-      int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
-      if (rc) return rc;
+      // the last segment's record came with the forward
+      const bool from_fwd = w.rec && sg == p.nseg - 1;
+      if (!from_fwd) {
+        int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+        if (rc) return rc;
+      }
+      aa.seg_lap = from_fwd ? w.rec : fa.rec_lap;
+      aa.seg_u = from_fwd ? w.rec + segcells : fa.rec_u;
       aa.seg_n0 = n0;
       for (int m = n1; m > n0;) {
         const int k = std::min(p.tb_a, m - n0);
@@ -1098,8 +1111,13 @@ int adj_nonlinear(const Plan &p, const float *b, const float *c, const float *rh
     const float *ck = w.ck + (size_t)sg * 2 * BF;
     fa.rec_n0 = n0;
     float *tmp[4] = {st[0], st[1], st[2], st[3]};
-    int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
-    if (rc) return rc;
+    const bool from_fwd = w.rec && sg == p.nseg - 1;
+    if (!from_fwd) {
+      int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+      if (rc) return rc;
+    }
+    aa.seg_lap = from_fwd ? w.rec : fa.rec_lap;
+    aa.seg_u = from_fwd ? w.rec + segcells : fa.rec_u;
     aa.seg_n0 = n0;
     nb.uprev0 = ck + BF;
     for (int m = n1; m > n0; --m) {
@@ -1163,7 +1181,16 @@ int wt_loop_fwd(void *plan, const float *b, const float *c, const float *rho, co
       o1 = w.ck + (size_t)(sg + 1) * 2 * BF;
     else
       o1 = (cur1 == bufs[4]) ? bufs[6] : bufs[4];
-    rc = run_forward(*p, a, n0, n1, cur1, cur2, o1, o1 + BF, tmp, false, s);
+    if (with_grad && w.rec && sg == p->nseg - 1) {
+      FwdArgs ar = a;
+      ar.rec_lap = w.rec;
+      ar.rec_u = p->nl ? w.rec + (size_t)p->B * p->K * p->Nx * (p->Ny + 1) : nullptr;
+      ar.rec_K = p->K;
+      ar.rec_n0 = n0;
+      rc = run_forward(*p, ar, n0, n1, cur1, cur2, o1, o1 + BF, tmp, true, s);
+    } else {
+      rc = run_forward(*p, a, n0, n1, cur1, cur2, o1, o1 + BF, tmp, false, s);
+    }
     if (rc) return rc;
     cur1 = o1;
     cur2 = o1 + BF;
@@ -1218,8 +1245,12 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
     const float *ck = w.ck + (size_t)sg * 2 * BF;
     fa.rec_n0 = n0;
     float *tmp[4] = {st[0], st[1], st[2], st[3]};
-    int rc;
-    if (use_tiled) {
+    // the last segment's record came with the forward when the forward ran
+    // the kernel family this sweep reads (same record layout)
+    const bool from_fwd = w.rec && sg == p->nseg - 1 && use_tiled == p->tiled;
+    int rc = WT_OK;
+    if (from_fwd) {
+    } else if (use_tiled) {
       rc = run_forward(*p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
     } else {
       Plan q = *p;
@@ -1227,6 +1258,7 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
       rc = run_forward(q, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
     }
     if (rc) return rc;
+    aa.seg_lap = from_fwd ? w.rec : w.seg;
     aa.seg_n0 = n0;
     // adjoint over m = n1 .. n0+1
     int m = n1;
