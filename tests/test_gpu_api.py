@@ -240,3 +240,33 @@ def test_training_step_does_not_synchronise(cuda):
     finally:
         torch.cuda.set_sync_debug_mode(0)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_single_segment_record_from_forward(cuda, nonlinear):
+    """With K >= T the whole run is one checkpoint segment: the forward writes
+    its record and the reverse sweep recomputes nothing.  Probe series equal
+    those of K = 16 (16 segments, 15 recomputed) bit for bit, and dJ/drho
+    agrees to fp32 regrouping of the per-launch gradient sums (1e-6)."""
+    def build(K):
+        torch.manual_seed(3)
+        geom = WaveGeometryFreeForm((56, 48), 1.0, c0=1.0, c1=0.5, abs_N=8, abs_sig=1.0,
+                                    abs_p=2.0, rho="rand")
+        nl = dict(satdamp_b0=0.5, satdamp_uth=0.3, c_nl=0.05) if nonlinear else {}
+        m = WaveRNN(WaveCell(0.5, geom, **nl), WaveSource(14, 24),
+                    [WaveIntensityProbe(40, 12), WaveIntensityProbe(40, 36)]).to(cuda)
+        m.checkpoint_every = K
+        return m
+    T = 240
+    n = np.arange(T)
+    x = torch.from_numpy((np.sin(0.2 * n) * np.exp(-((n - 80) / 30.0) ** 2))
+                         .astype(np.float32))[None].repeat(3, 1).to(cuda)
+    out = {}
+    for K in (T, 16):
+        m = build(K)
+        y = m(x)
+        (y * torch.linspace(0.5, 1.5, y.numel(), device=cuda).reshape(y.shape)).sum().backward()
+        out[K] = (y.detach().cpu(), m.cell.geom.rho.grad.detach().cpu())
+        assert next(iter(m._plans.values())).tuning["ck_every"] == K
+    assert torch.equal(out[T][0], out[16][0])
+    assert rel(out[T][1].numpy(), out[16][1].numpy()) < 1e-6
