@@ -147,7 +147,9 @@ int wt_plan_tuning(const void *plan, wt_tuning *out);
 
 /* Bytes of the two caller-provided workspaces:
  *   saved: lives from wt_loop_fwd to wt_loop_adj (coefficients, checkpoints,
- *          probe amplitudes); needed only when with_grad.
+ *          probe amplitudes, and the record of the last checkpoint segment,
+ *          which the forward writes so that the reverse sweep need not
+ *          recompute that segment); needed only when with_grad.
  *   temp:  scratch of one call (state ping-pong, adjoint segment buffer,
  *          adjoint state, gradient partials). */
 int wt_workspace_bytes(const void *plan, int32_t with_grad, int64_t *saved,
@@ -161,13 +163,15 @@ int wt_workspace_bytes(const void *plan, int32_t with_grad, int64_t *saved,
  *   fields      (B,T/field_stride,Nx,Ny) fields u_{n+1} for n+1 a multiple of
  *               field_stride (rnn.py:66 with output_fields; stride 1 =
  *               reference behaviour), may be NULL
- *   with_grad   store checkpoints/amplitudes into `saved` for wt_loop_adj */
+ *   with_grad   store checkpoints, amplitudes and the last segment's record
+ *               into `saved` for wt_loop_adj */
 int wt_loop_fwd(void *plan, const float *b, const float *c, const float *rho,
                 const float *x, float *y, float *fields, int32_t field_stride,
                 int32_t with_grad, void *saved, void *temp, void *stream);
 
 /* Reverse sweep (the autograd replay of cell.py:26-44 over rnn.py:50-70)
- * with checkpoint recompute every K steps.
+ * with checkpoint recompute every K steps (every segment but the last, whose
+ * record the forward left in `saved`; with K >= T nothing is recomputed).
  *   ybar        (B,T,ntap) dJ/dy, or NULL
  *   fbar        (B,T,Nx,Ny) dJ/d(fields) (stride-1 fields only), or NULL
  *   grad_c      (Nx,Ny) dJ/dc_linear, summed over batch, or NULL

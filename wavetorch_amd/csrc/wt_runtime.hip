@@ -1248,16 +1248,17 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
     // the last segment's record came with the forward when the forward ran
     // the kernel family this sweep reads (same record layout)
     const bool from_fwd = w.rec && sg == p->nseg - 1 && use_tiled == p->tiled;
-    int rc = WT_OK;
-    if (from_fwd) {
-    } else if (use_tiled) {
-      rc = run_forward(*p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
-    } else {
-      Plan q = *p;
-      q.tiled = false;
-      rc = run_forward(q, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+    if (!from_fwd) {
+      int rc;
+      if (use_tiled) {
+        rc = run_forward(*p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+      } else {
+        Plan q = *p;
+        q.tiled = false;
+        rc = run_forward(q, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+      }
+      if (rc) return rc;
     }
-    if (rc) return rc;
     aa.seg_lap = from_fwd ? w.rec : w.seg;
     aa.seg_n0 = n0;
     // adjoint over m = n1 .. n0+1
