@@ -182,6 +182,8 @@ class GraphedTrainStep:
         with torch.no_grad():  # undo the warm-up steps
             for p, v in zip(params, p0):
                 p.copy_(v)
+                if p.grad is not None:  # the graph's gradient buffer: defined
+                    p.grad.zero_()      # before the first replay
             for p in params:
                 old = st0.get(p, {})
                 for k, v in optimizer.state.get(p, {}).items():
