@@ -9,7 +9,8 @@ history with the reference's columns and, given a ``name``, saves the model
 (``io.save_model``).  The criterion is evaluated on
 ``normalize_power(model(xb).sum(dim=1))`` as in train.py:61-62; for the
 default ``torch.nn.CrossEntropyLoss`` the fused loss head
-(``loss.power_cross_entropy``) computes the same value in one kernel pair.
+(``loss.power_cross_entropy``) computes the same value in three short kernel
+launches.
 
 ``GraphedTrainStep`` captures forward, loss head, checkpointed adjoint,
 optimizer step and design-region constraint of one batch shape in a single
