@@ -127,7 +127,9 @@ typedef struct wt_tuning {
 } wt_tuning;
 
 /* Creates a plan for `cfg` at batch B and length T: validates the
- * configuration (indices, CFL as cell.py:66-73 with cmax from `cmax`),
+ * configuration (indices, CFL as cell.py:66-73 with cmax from `cmax`; grids
+ * up to Nx * (Ny + 1) < 2^22 cells, about 2047 x 2047, which the kernels'
+ * 32-bit in-field cell indices assume -- WT_EUNSUPPORTED beyond),
  * picks the kernel variant and uploads the source/probe/tile tables to the
  * current device (synchronously).  The plan is library-owned; destroy with
  * wt_plan_destroy.  `tuning` may be NULL. */
