@@ -49,7 +49,7 @@ def lib():
         L.wo_forward_f64.argtypes = [i, i, i, i, P, P, d, d, P, i, P, i, P, P, P, i, i, P,
                                      d, d, d, d, d]
         L.wo_adjoint_f64.argtypes = [i, i, i, i, P, P, d, d, P, i, P, i, P, i, P,
-                                     d, d, d, d, d, P, P, P, P, P, P]
+                                     d, d, d, d, d, P, P, P, P, P, P, P]
         for fn in (L.wo_forward_f32, L.wo_forward_f64, L.wo_adjoint_f64):
             fn.restype = ctypes.c_int
         _lib = L
@@ -123,8 +123,13 @@ def forward(b, c, x, dt, h, src_px, taps, fields=False, nl=None, dtype=np.float3
     return y, fo
 
 
-def adjoint(b, c, x, dt, h, src_px, taps, ybar=None, fbar=None, nl=None):
-    """fp64 discrete adjoint.  Returns dict(gc, gb, grho, gx)."""
+def adjoint(b, c, x, dt, h, src_px, taps, ybar=None, fbar=None, nl=None, at_fields=None):
+    """fp64 discrete adjoint.  Returns dict(gc, gb, grho, gx).
+
+    at_fields: None (the adjoint of the fp64 forward), or the fp32 fields
+    u_1..u_T, shaped (B, T, Nx, Ny), of a forward computed elsewhere: the
+    fp64 reverse sweep is then taken at that trajectory (the exact adjoint of
+    the computed forward)."""
     b = _c(b, np.float64)
     c = _c(c, np.float64)
     x = _c(x, np.float64)
@@ -140,9 +145,14 @@ def adjoint(b, c, x, dt, h, src_px, taps, ybar=None, fbar=None, nl=None):
     gb = np.zeros((Nx, Ny))
     grho = np.zeros((Nx, Ny))
     gx = np.zeros((B, T))
+    u32 = None
+    if at_fields is not None:
+        u32 = np.ascontiguousarray(at_fields, dtype=np.float32)
+        if u32.shape != (B, T, Nx, Ny):
+            raise ValueError(f"at_fields shape {u32.shape}, expected {(B, T, Nx, Ny)}")
     rc = lib().wo_adjoint_f64(B, Nx, Ny, T, _p(b), _p(c), dt, h, _p(x), len(src_px), _p(src_px),
                               len(taps), _p(taps), flags, _p(rho), sb0, suth, cnl, cmin, cmax,
-                              _p(ybar), _p(fbar), _p(gc), _p(gb), _p(grho), _p(gx))
+                              _p(ybar), _p(fbar), _p(u32), _p(gc), _p(gb), _p(grho), _p(gx))
     if rc != 0:
         raise MemoryError("wo_adjoint failed")
     return dict(gc=gc, gb=gb, grho=grho, gx=gx)

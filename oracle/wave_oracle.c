@@ -188,6 +188,12 @@ DEF_FWD(double, f64)
  *   gb     : dJ/db        (Nx,Ny), summed over batch
  *   grho   : dJ/drho through the nonlinear terms only (Nx,Ny)
  *   gx     : dJ/dx (B,T)
+ *   u32    : NULL, or the fields u_1..u_T (B,T,Nx,Ny) of a forward computed
+ *            elsewhere (wo_forward_f32 with fstride 1): the reverse sweep then
+ *            runs in fp64 at that trajectory instead of its own fp64 forward,
+ *            i.e. the exact adjoint of the computed forward (the step's
+ *            Jacobian depends on u through the probe seeds 2*u*ybar, the c
+ *            gradient's lap(u) and the nonlinear coefficients)
  * Memory: keeps every forward field (small problems only).
  */
 int wo_adjoint_f64(int B, int Nx, int Ny, int Tn, const double *bfield,
@@ -196,7 +202,8 @@ int wo_adjoint_f64(int B, int Nx, int Ny, int Tn, const double *bfield,
                    const int32_t *taps, int nl, const double *rho,
                    double sat_b0, double sat_uth, double c_nl, double cmin,
                    double cmax, const double *ybar, const double *fbar,
-                   double *gc, double *gb, double *grho, double *gx) {
+                   const float *u32, double *gc, double *gb, double *grho,
+                   double *gx) {
   size_t F = (size_t)Nx * Ny;
   /* U[n] = u_n for n = -1..Tn  -> index n+1 */
   double *U = (double *)calloc((size_t)(Tn + 2) * B * F, sizeof(double));
@@ -221,8 +228,15 @@ int wo_adjoint_f64(int B, int Nx, int Ny, int Tn, const double *bfield,
   int maxsrc = -1;
   for (int k = 0; k < nsrcpx; ++k)
     if (src_px[3 * k + 2] > maxsrc) maxsrc = src_px[3 * k + 2];
-  /* forward, storing every field */
-  for (int n = 0; n < Tn; ++n) {
+  /* forward, storing every field (or the given trajectory, upcast) */
+  if (u32)
+    for (int n = 0; n < Tn; ++n)
+      for (int b = 0; b < B; ++b) {
+        const float *src = u32 + ((size_t)b * Tn + n) * F;
+        double *dst = UU(n + 1, b);
+        for (size_t q = 0; q < F; ++q) dst[q] = (double)src[q];
+      }
+  for (int n = 0; n < Tn && !u32; ++n) {
     for (int b = 0; b < B; ++b) {
       const double *y1 = UU(n, b), *y2 = UU(n - 1, b);
       double *yn = UU(n + 1, b);

@@ -126,8 +126,79 @@ def main(root="/root/reference"):
                loop_src=np.array([[src[0], src[1], 0]], dtype=np.int32),
                loop_taps=np.array([[22, 8, 1], [22, 13, 1], [21, 18, 0]], dtype=np.int32),
                loop_dt=np.float32(0.707))
+    out.update(nonlinear_loops(cellm, probem))
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, {k: v.shape for k, v in out.items()})
+
+
+class _Geometry:
+    """The three attributes WaveCell reads from its geometry (cell.py:66-67,
+    94-105): b, h and cmax.  geom.py itself needs scikit-image, absent here."""
+
+    def __init__(self, b, h, cmax):
+        self.b, self.h, self.cmax = b, h, cmax
+
+    def parameters(self):
+        return iter(())
+
+
+# (name, satdamp_b0, satdamp_uth, c_nl): the saturable-damping and Kerr-speed
+# branches of cell.py:94-102 alone and together, at the nonlinear training
+# configuration's values (b_nl = 0.5, I_sat = 0.1 -> uth = sqrt(0.1), c_nl = 0.05)
+NL_CASES = (("nlsat", 0.5, 0.1 ** 0.5, 0.0), ("nlkerr", 0.0, 0.0, 0.05),
+            ("nlboth", 0.5, 0.1 ** 0.5, 0.05))
+
+
+def nonlinear_loops(cellm, probem):
+    """A 48-step loop of the reference's own WaveCell.forward (cell.py:79-107)
+    with the nonlinear terms on, driven in the order of rnn.py:39-70: u_0 =
+    u_-1 = 0; per step (h1, h2) = cell(h1, h2, c_linear, rho), then the source
+    sample added as source.py:19-22 does, then the probes.  rho is the raw
+    density (rnn.py:47 passes geom.rho).  Recorded per case: the probe series,
+    the last two fields, and the reference's autograd gradients of a weighted
+    probe sum with respect to c_linear, the PML field b, rho and x."""
+    g = torch.Generator().manual_seed(20240602)
+    T, Nx, Ny, B = 48, 30, 26, 2
+    dt = 0.6
+    b0 = pml(Nx, Ny, 5, 1.0, 2)
+    c = torch.ones(Nx, Ny)
+    c[6:24, 6:20] = 0.5 + 0.5 * torch.rand(18, 14, generator=g)
+    rho = torch.zeros(Nx, Ny)
+    rho[6:24, 6:20] = torch.rand(18, 14, generator=g)
+    n = torch.arange(T, dtype=torch.float32)
+    x = torch.stack([1.5 * torch.sin(2 * np.pi * 0.05 * n + 0.4 * k) for k in range(B)])
+    src = (9, 13)
+    probes = [probem.WaveIntensityProbe(22, 8), probem.WaveIntensityProbe(22, 13),
+              probem.WaveProbe(21, 18)]
+    w = torch.randn(B, T, len(probes), generator=g)
+    out = dict(nl_b=b0.numpy(), nl_c=c.numpy(), nl_rho=rho.numpy(), nl_x=x.numpy(),
+               nl_w=w.numpy(), nl_dt=np.float32(dt),
+               nl_src=np.array([[src[0], src[1], 0]], dtype=np.int32),
+               nl_taps=np.array([[22, 8, 1], [22, 13, 1], [21, 18, 0]], dtype=np.int32))
+    for name, sb0, suth, cnl in NL_CASES:
+        bg = b0.clone().requires_grad_(True)
+        cg = c.clone().requires_grad_(True)
+        rg = rho.clone().requires_grad_(True)
+        xg = x.clone().requires_grad_(True)
+        cell = cellm.WaveCell(dt, _Geometry(bg, torch.tensor(1.0), 1.0), satdamp_b0=sb0,
+                              satdamp_uth=suth, c_nl=cnl)
+        h1 = torch.zeros(B, Nx, Ny)
+        h2 = torch.zeros(B, Nx, Ny)
+        ys = []
+        for xi in xg.chunk(T, dim=1):
+            h1, h2 = cell(h1, h2, cg, rg)
+            X_expanded = torch.zeros(h1.size()).detach()
+            X_expanded[:, torch.tensor(src[0]), torch.tensor(src[1])] = xi.squeeze(-1)
+            h1 = h1 + 1.0 ** 2 * X_expanded
+            ys.append(torch.stack([p(h1) for p in probes], dim=-1))
+        yall = torch.stack(ys, dim=1)
+        (yall * w).sum().backward()
+        out.update({f"{name}_y": yall.detach().numpy(), f"{name}_u1": h1.detach().numpy(),
+                    f"{name}_u2": h2.detach().numpy(), f"{name}_gb": bg.grad.numpy(),
+                    f"{name}_gc": cg.grad.numpy(), f"{name}_grho": rg.grad.numpy(),
+                    f"{name}_gx": xg.grad.numpy(),
+                    f"{name}_par": np.array([sb0, suth, cnl], dtype=np.float64)})
+    return out
 
 
 if __name__ == "__main__":

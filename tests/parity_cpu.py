@@ -100,3 +100,38 @@ def task_lens(sp, cg, ybar):
                         dtype=np.float64)
     g = co.adjoint(sp["b"], c64, sp["x"], sp["dt"], sp["h"], sp["src"], sp["taps"], ybar=ybar)
     return y32, y64, g["gc"], g["gx"]
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a))
+
+
+def task_lens_fwd32(sp, cg, yg):
+    """One batched instance: is the GPU's probe series `yg` (T, P) bit for bit
+    the fp32 oracle's, given the GPU's c?"""
+    y32, _ = co.forward(sp["b"].astype(np.float32), cg, sp["x"].astype(np.float32), sp["dt"],
+                        sp["h"], sp["src"], sp["taps"])
+    return bool(np.array_equal(yg, y32[0]))
+
+
+def task_lens_check(sp, cg, yg, gcg, gxg):
+    """One batched instance with J = -sum of its focal intensity: the forward
+    bit for bit against the fp32 oracle given the GPU's c, and the GPU's
+    dJ/dc, dJ/dx relative to the fp64 adjoint taken at the fp32 trajectory
+    (`_at`: the exact adjoint of the computed forward) and at the fp64
+    forward's (`64`)."""
+    T = sp["x"].shape[1]
+    ybar = -np.ones((1, T, 1))
+    y32, f32 = co.forward(sp["b"].astype(np.float32), cg, sp["x"].astype(np.float32), sp["dt"],
+                          sp["h"], sp["src"], sp["taps"], fields=True)
+    c64 = cg.astype(np.float64)
+    g_at = co.adjoint(sp["b"], c64, sp["x"], sp["dt"], sp["h"], sp["src"], sp["taps"],
+                      ybar=ybar, at_fields=f32)
+    del f32
+    g64 = co.adjoint(sp["b"], c64, sp["x"], sp["dt"], sp["h"], sp["src"], sp["taps"], ybar=ybar)
+    return dict(bitwise=bool(np.array_equal(yg, y32[0])),
+                gc_at=_rel(gcg, g_at["gc"]), gx_at=_rel(gxg, g_at["gx"][0]),
+                gc64=_rel(gcg, g64["gc"]), gx64=_rel(gxg, g64["gx"][0]))

@@ -1,0 +1,228 @@
+"""GPU parity at the full sizes of three configurations:
+
+* nonlinear 150x150, batch 32, 3000 steps (saturable damping + clamped Kerr
+  speed): the temporally blocked forward equals the one-launch-per-step
+  forward bit for bit over the whole batch and length, and equals the fp32
+  C oracle bit for bit for four samples at the full length (the GPU's c);
+  gradients with respect to c, rho and x of the whole batch within 1e-4 of
+  the fp64 oracle over the first 300 steps (past ~400 steps the exact
+  gradient of this batch is ill-conditioned: test_oracle.py pins that);
+* the 256 lens instances (sides 64-256 with a 16-cell PML, 1000-4000 steps)
+  as one batch with the tuning the benchmark uses: every instance's probe
+  series bit for bit against the fp32 C oracle; for the longest instance of
+  each size, dJ/dc and dJ/dx within 1e-4 of the fp64 adjoint taken at the
+  computed (fp32) trajectory, and within 1e-2 of the fp64 adjoint of the
+  fp64 forward (over 4000 steps fp32 rounding of the forward alone moves the
+  gradient by up to ~3e-3 relative, in the reference's fp32 path as much as
+  here);
+* the training step's free-form chain rho -> blur -> projection -> c -> loop
+  -> cross-entropy -> adjoint -> rho at 150x150, batch 4, 3000 steps, for
+  beta = 100 (every configuration's value) and beta = 1000 (the lens study
+  steps beta up to 2000): forward probe series within 1e-5 of the
+  reference's fp32 op sequence (oracle/ref_torch.py), dJ/drho within 1e-4 of
+  the reference's fp32 autograd gradient and of the fp64 oracle chain, and
+  within 1e-4 of fp64 central differences on 8 pixels.
+
+The CPU legs (tests/parity_cpu.py: numpy and the C oracle) run in spawned
+worker processes, which never load the GPU stack.
+"""
+import multiprocessing as mp
+import os
+import time
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+import pytest
+import torch
+
+from tests import parity_cpu as pc
+from tests.helpers import rel
+from tests.parity_report import _chunks, c_map_vjp, loss64, ref32_grad, spec_of
+from wavetorch_amd import autotune, workloads
+from wavetorch_amd._loop import Plan, WaveLoop
+from wavetorch_amd.utils import normalize_power
+
+pytestmark = pytest.mark.gpu
+
+WORKERS = max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.fixture(scope="module")
+def pool():
+    ex = ProcessPoolExecutor(WORKERS, mp_context=mp.get_context("spawn"))
+    yield ex
+    ex.shutdown(wait=True, cancel_futures=True)
+
+
+def _log(*a):
+    print(*a, flush=True)
+
+
+def _loop(cfg, B, T, cuda, tun, c, b, x, rho=None, grad=False):
+    plan = Plan(cfg, B, T, 1.0, cuda, tun)
+    ct = c.clone().requires_grad_(grad)
+    xt = x.clone().requires_grad_(grad)
+    rt = rho.clone().requires_grad_(grad) if rho is not None else None
+    with torch.set_grad_enabled(grad):
+        y = WaveLoop.apply(xt, ct, b, rt, plan, False, 1)
+    return plan, y, ct, xt, rt
+
+
+# ---------------------------------------------------------------- nonlinear
+
+NL_SAMPLES = (0, 1, 26, 31)
+NL_GRAD_STEPS = 300
+
+
+def test_nonlinear_150_b32_full_length(cuda, pool):
+    model, x, _ = workloads.nonlinear_150_b32()
+    model, x = model.to(cuda), x.to(cuda)
+    cfg = model.loop_config()
+    B, T = x.shape
+    geom = model.cell.geom
+    with torch.no_grad():
+        c, b, rho = geom.c.detach().clone(), geom.b.detach().clone(), geom.rho.detach().clone()
+    sp = spec_of(model, x, None)
+    cg, rho32 = c.cpu().numpy(), rho.cpu().numpy()
+    jobs = {s: pool.submit(pc.task_fwd32, sp, cg, rho32, slice(s, s + 1)) for s in NL_SAMPLES}
+    tuned = dict(autotune.cached("nonlinear_150_b32") or {})
+    ys = {}
+    for tun in (tuned, {"plain": 1}):
+        _log("case nonlinear full forward", tun)
+        plan, y, *_ = _loop(cfg, B, T, cuda, {**tun, "ck_every": 64}, c, b, x, rho)
+        ys[plan.tuning["plain"]] = y.cpu().numpy()
+        del plan, y
+    assert np.isfinite(ys[0]).all() and np.abs(ys[0]).max() > 0
+    assert np.array_equal(ys[0], ys[1])
+    for s, j in jobs.items():
+        assert np.array_equal(ys[0][s:s + 1], j.result()), f"sample {s}"
+
+    # gradients of the whole batch over the first steps vs the fp64 oracle
+    Tg = NL_GRAD_STEPS
+    xg = x[:, :Tg].contiguous()
+    w = np.random.default_rng(12).standard_normal((B, Tg, 3))
+    sp_g = dict(sp, x=sp["x"][:, :Tg])
+    parts = [pool.submit(pc.task_adj64, sp_g, rho32.astype(np.float64), w, ch,
+                         cg.astype(np.float64)) for ch in _chunks(B, WORKERS)]
+    _log("case nonlinear gradients", Tg, "steps")
+    plan, y, ct, xt, rt = _loop(cfg, B, Tg, cuda, {**tuned, "ck_every": 64}, c, b, xg, rho,
+                                grad=True)
+    (y * torch.from_numpy(w.astype(np.float32)).to(cuda)).sum().backward()
+    res = [p.result() for p in parts]
+    gc64 = sum(r[0] for r in res)
+    grho64 = sum(r[1] for r in res)
+    gx64 = np.concatenate([r[2] for r in res], axis=0)
+    errs = {"c": rel(ct.grad.cpu().numpy(), gc64), "rho": rel(rt.grad.cpu().numpy(), grho64),
+            "x": rel(xt.grad.cpu().numpy(), gx64)}
+    _log("nonlinear gradient rel errors vs fp64", errs)
+    assert max(errs.values()) < 1e-4, errs
+
+
+# ---------------------------------------------------------------- lens batch
+
+def test_lens_batch_256_full(cuda, pool):
+    from wavetorch_amd.batch import Instance, InstanceBatch
+    items = workloads.lens_batch_256()
+    models = [m.to(cuda) for m, _ in items]
+    xs = [x.to(cuda) for _, x in items]
+    with torch.no_grad():
+        cs = [m.cell.geom.c.detach().clone() for m in models]
+    ybars = [-torch.ones((x.shape[0], 1), device=cuda) for x in xs]
+    insts = [Instance.from_model(m, x.shape[0]) for m, x in zip(models, xs)]
+    bt = InstanceBatch(insts, cuda, tuning=autotune.cached_batch(insts))
+    bt.set_inputs(cs, xs, ybars)
+    _log("case lens batch: 256 instances forward + adjoint", bt.tuning[0])
+    bt.fwd_adj()
+    torch.cuda.synchronize(cuda)
+    outs = [(y.cpu().numpy(), gc.cpu().numpy(), gx.cpu().numpy()) for y, gc, gx in bt.outputs()]
+    sides = [int(m.cell.geom.b.shape[0]) for m in models]
+    lengths = [int(x.shape[0]) for x in xs]
+    # the longest instance of each size gets the gradient check
+    rep = {}
+    for i, (n, t) in enumerate(zip(sides, lengths)):
+        if n not in rep or t > lengths[rep[n]]:
+            rep[n] = i
+    specs = [spec_of(m, x[None], None) for m, x in zip(models, xs)]
+    grad_jobs = {i: pool.submit(pc.task_lens_check, specs[i], cs[i].cpu().numpy(),
+                                outs[i][0], outs[i][1], outs[i][2]) for i in rep.values()}
+    fwd_jobs = {i: pool.submit(pc.task_lens_fwd32, specs[i], cs[i].cpu().numpy(), outs[i][0])
+                for i in range(len(items)) if i not in grad_jobs}
+    t0 = time.time()
+    bad = [i for i, j in fwd_jobs.items() if not j.result()]
+    _log(f"lens forward: {len(fwd_jobs) - len(bad)} of {len(fwd_jobs)} bitwise "
+         f"({time.time() - t0:.0f} s)")
+    assert not bad, f"instances not bitwise: {bad[:10]}"
+    for i, j in grad_jobs.items():
+        r = j.result()
+        _log(f"lens instance {i} ({sides[i]}^2, {lengths[i]} steps): {r}")
+        assert r["bitwise"], i
+        assert r["gc_at"] < 1e-4 and r["gx_at"] < 1e-4, (i, r)
+        assert r["gc64"] < 1e-2 and r["gx64"] < 1e-2, (i, r)
+
+
+# ---------------------------------------------------------------- rho chain
+
+# central-difference step: 1e-5 at beta = 100, scaled by 100 / beta so that
+# the truncation error (~ delta^2 beta^3 of the projection) stays small
+def _fd_delta(beta):
+    return 1e-5 * min(1.0, 100.0 / beta)
+
+
+def _gpu_train_grad(model, x, labels, cuda):
+    model = model.to(cuda)
+    model.zero_grad(set_to_none=True)
+    y = model(x.to(cuda))
+    J = torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels.to(cuda))
+    J.backward()
+    torch.cuda.synchronize(cuda)
+    return y.detach().cpu().numpy(), model.cell.geom.rho.grad.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("beta", [100.0, 1000.0])
+def test_freeform_chain_train_150_b4(beta, cuda, pool):
+    model, x, labels = workloads.train_150_b64(B=4)
+    with torch.no_grad():
+        model.cell.geom.beta.fill_(beta)
+    sp = spec_of(model, x, labels)
+    rho64 = model.cell.geom.rho.detach().cpu().numpy().astype(np.float64)
+    B = x.shape[0]
+    # fp64 chain: forward per sample, loss, adjoint, then through rho -> c
+    fwd = [pool.submit(pc.task_fwd64, sp, rho64, s) for s in _chunks(B, B)]
+    _log(f"case rho chain beta={beta}: GPU")
+    yg, grho_g = _gpu_train_grad(model, x, labels, cuda)
+    y64 = np.concatenate([f.result() for f in fwd], axis=0)
+    J64, ybar = loss64(sp, y64)
+    adj = [pool.submit(pc.task_adj64, sp, rho64, ybar, s) for s in _chunks(B, B)]
+    # central differences on the pixels with the largest fp64-chain gradient
+    # magnitudes among a seeded draw of design-region pixels (at beta = 1000
+    # most pixels' gradient is ~0, where a relative check says nothing)
+    res = [a.result() for a in adj]
+    grho64 = c_map_vjp(sp, rho64, sum(r[0] for r in res)) + sum(r[1] for r in res)
+    ii, jj = np.nonzero(sp["b"] == 0)
+    draw = np.random.default_rng(7).choice(len(ii), size=256, replace=False)
+    cand = sorted(draw, key=lambda k: -abs(grho64[ii[k], jj[k]]))[:8]
+    pixels = [(int(ii[k]), int(jj[k])) for k in cand]
+    delta = _fd_delta(beta)
+    fd_jobs = []
+    for (i, j) in pixels:
+        for sgn in (1.0, -1.0):
+            r = rho64.copy()
+            r[i, j] += sgn * delta
+            fd_jobs.append(pool.submit(pc.task_J64, sp, r))
+    _log(f"case rho chain beta={beta}: reference fp32 autograd")
+    torch.set_num_threads(WORKERS)
+    yr, grho_r = ref32_grad(model.cpu(), x, labels)
+    Js = [f.result() for f in fd_jobs]
+    fd = np.array([(Js[2 * k] - Js[2 * k + 1]) / (2 * delta) for k in range(len(pixels))])
+    gpx = np.array([grho_g[i, j] for (i, j) in pixels])
+    opx = np.array([grho64[i, j] for (i, j) in pixels])
+    errs = {"fwd_vs_ref32": rel(yg, yr), "fwd_vs_fp64": rel(yg, y64),
+            "ref32_fwd_vs_fp64": rel(yr, y64),
+            "grad_vs_ref32": rel(grho_g, grho_r), "grad_vs_fp64": rel(grho_g, grho64),
+            "ref32_grad_vs_fp64": rel(grho_r, grho64),
+            "grad_vs_fd": rel(gpx, fd), "fp64_vs_fd": rel(opx, fd)}
+    _log(f"rho chain beta={beta}:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert errs["fp64_vs_fd"] < 1e-5, errs  # the checker itself
+    assert errs["fwd_vs_ref32"] < 1e-5, errs
+    assert errs["grad_vs_ref32"] < 1e-4 and errs["grad_vs_fp64"] < 1e-4, errs
+    assert errs["grad_vs_fd"] < 1e-4, errs

@@ -7,6 +7,8 @@ kernel family.  Adjoint: dJ/dc, dJ/db, dJ/drho and dJ/dx within 1e-4 relative
 of the fp64 oracle adjoint (itself pinned by central finite differences and
 by torch autograd through the restated reference cell in test_oracle.py), over
 several checkpoint intervals so that segment boundaries fall mid-run."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -51,6 +53,53 @@ def _oracle_nl(p, case):
     nl = dict(CASES[case])
     nl["rho"] = p["rho"]
     return nl
+
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with np.load(GOLD, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("name", ["nlsat", "nlkerr", "nlboth"])
+@pytest.mark.parametrize("tun,K", [({"plain": 1}, 16), ({}, 20), ({"kind": 1}, 48),
+                                   ({"kind": 0, "strips": 2, "tcols": 2, "tblock": 8,
+                                     "nt": 1024, "cpt": 8}, 13)],
+                         ids=["plain-K16", "auto-K20", "flat-K48", "tiles2x2-K13"])
+def test_nonlinear_reference_golden(name, tun, K, gold, cuda):
+    """Against a 48-step loop of the reference's own WaveCell.forward
+    (cell.py:79-107, tests/golden/make_golden.py): probe series and the last
+    two fields bit for bit, and the reference's autograd gradients with
+    respect to c_linear, b, raw rho and x within 1e-4."""
+    sb0, suth, cnl = (float(v) for v in gold[f"{name}_par"])
+    B, T = gold["nl_x"].shape
+    Nx, Ny = gold["nl_b"].shape
+    cfg = LoopConfig(Nx=Nx, Ny=Ny, dt=float(gold["nl_dt"]), h=1.0,
+                     src_px=tuple(map(tuple, gold["nl_src"].tolist())),
+                     taps=tuple(map(tuple, gold["nl_taps"].tolist())),
+                     sat_b0=sb0, sat_uth=suth, c_nl=cnl)
+    print("case", name, tun, K, flush=True)
+    plan = Plan(cfg, B, T, 1.0, cuda, {**tun, "ck_every": K})
+    t = {k: torch.from_numpy(gold[f"nl_{k}"]).to(cuda).requires_grad_(True)
+         for k in ("x", "c", "b", "rho")}
+    y = WaveLoop.apply(t["x"], t["c"], t["b"], t["rho"], plan, False, 1)
+    assert np.array_equal(y.detach().cpu().numpy(), gold[f"{name}_y"]), plan.tuning
+    (y * torch.from_numpy(gold["nl_w"]).to(cuda)).sum().backward()
+    for key, arg in (("gc", "c"), ("gb", "b"), ("grho", "rho"), ("gx", "x")):
+        ref = gold[f"{name}_{key}"]
+        got = t[arg].grad.cpu().numpy()
+        if not np.any(ref):  # the Kerr-free cell has no rho dependence
+            assert np.abs(got).max() == 0.0, key
+            continue
+        assert rel(got, ref) < 1e-4, (key, rel(got, ref), plan.tuning)
+    with torch.no_grad():
+        f = WaveLoop.apply(t["x"], t["c"], t["b"], t["rho"], plan, True, 1)
+    f = f.cpu().numpy()
+    assert np.array_equal(f[:, -1], gold[f"{name}_u1"]), plan.tuning
+    assert np.array_equal(f[:, -2], gold[f"{name}_u2"]), plan.tuning
 
 
 @pytest.mark.parametrize("case", list(CASES))

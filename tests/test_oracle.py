@@ -84,6 +84,80 @@ def test_loop_golden_bitwise(gold):
     assert np.array_equal(y, gold["loop_y"])
 
 
+NL_GOLD = ("nlsat", "nlkerr", "nlboth")
+
+
+def gold_nl(gold, name, rho=None):
+    """Oracle nonlinear arguments of golden case `name` (cell.py:94-102 with
+    the recorded satdamp_b0, satdamp_uth, c_nl); `rho` overrides the density."""
+    sb0, suth, cnl = (float(v) for v in gold[f"{name}_par"])
+    return dict(rho=gold["nl_rho"] if rho is None else rho, sat_b0=sb0, sat_uth=suth, c_nl=cnl)
+
+
+def _nl_forward(gold, nl):
+    return co.forward(gold["nl_b"], gold["nl_c"], gold["nl_x"], float(gold["nl_dt"]), 1.0,
+                      gold["nl_src"], gold["nl_taps"], fields=True, nl=nl)
+
+
+@pytest.mark.parametrize("name", NL_GOLD)
+def test_nonlinear_loop_golden_bitwise(gold, name):
+    """wo_forward_f32 with saturable damping and/or the Kerr speed equals a
+    48-step loop of the reference's own WaveCell.forward (cell.py:79-107) bit
+    for bit: probe series and the last two fields."""
+    y, f = _nl_forward(gold, gold_nl(gold, name))
+    assert np.array_equal(y, gold[f"{name}_y"])
+    assert np.array_equal(f[:, -1], gold[f"{name}_u1"])
+    assert np.array_equal(f[:, -2], gold[f"{name}_u2"])
+
+
+@pytest.mark.parametrize("name", NL_GOLD)
+def test_nonlinear_golden_rejects_plausible_mistakes(gold, name):
+    """The nonlinear golden vectors tell apart the assembly of cell.py:94-102
+    from the mistakes a restatement could share with the kernels: the rho
+    weight dropped (rho = 1), the projected instead of the raw density
+    (P(rho) at beta = 100), and rho applied squared."""
+    rho = gold["nl_rho"].astype(np.float32)
+    proj = ((1.0 + np.tanh(100.0 * (rho.astype(np.float64) - 0.5))) / 2.0).astype(np.float32)
+    for bad in (np.ones_like(rho), proj, rho * rho):
+        y, _ = _nl_forward(gold, gold_nl(gold, name, rho=bad))
+        assert rel(y, gold[f"{name}_y"]) > 1e-3
+
+
+@pytest.mark.parametrize("name", NL_GOLD)
+def test_nonlinear_adjoint_golden(gold, name):
+    """wo_adjoint_f64 against the reference's autograd gradients of the same
+    nonlinear loop (fp32 autograd through cell.py:26-44 and the nonlinear
+    assembly) with respect to c_linear, the PML field b, raw rho and x."""
+    g = co.adjoint(gold["nl_b"], gold["nl_c"], gold["nl_x"], float(gold["nl_dt"]), 1.0,
+                   gold["nl_src"], gold["nl_taps"], ybar=gold["nl_w"], nl=gold_nl(gold, name))
+    for key in ("gc", "gb", "grho", "gx"):
+        ref = gold[f"{name}_{key}"]
+        if key == "grho" and not np.any(ref):
+            continue
+        assert rel(g[key], ref) < 1e-5, (key, rel(g[key], ref))
+
+
+def test_adjoint_at_given_trajectory():
+    """wo_adjoint_f64 taken at a given fp32 trajectory: at the fp64 forward's
+    own fields (rounded to fp32) it equals the plain fp64 adjoint to fp32
+    rounding, linear and nonlinear; a trajectory one step out of place (the
+    index mistake this mode could make) is far off."""
+    p = small_problem(T=150)
+    rho = np.random.default_rng(5).uniform(0, 1, (p["Nx"], p["Ny"]))
+    w = np.random.default_rng(1).standard_normal((p["B"], p["T"], 3))
+    for nl in (None, dict(rho=rho, sat_b0=0.5, sat_uth=0.1 ** 0.5, c_nl=0.05)):
+        args = (p["b"], p["c"], p["x"], 0.6, 1.0, p["src"], p["taps"])
+        _, f64 = co.forward(*args, fields=True, dtype=np.float64, nl=nl)
+        g0 = co.adjoint(*args, ybar=w, nl=nl)
+        g1 = co.adjoint(*args, ybar=w, nl=nl, at_fields=f64.astype(np.float32))
+        shifted = np.concatenate([np.zeros_like(f64[:, :1]), f64[:, :-1]], axis=1)
+        g2 = co.adjoint(*args, ybar=w, nl=nl, at_fields=shifted.astype(np.float32))
+        keys = ("gc", "gx") if nl is None else ("gc", "gb", "grho", "gx")
+        for k in keys:
+            assert rel(g1[k], g0[k]) < 1e-5, (k, rel(g1[k], g0[k]))
+        assert max(rel(g2[k], g0[k]) for k in keys) > 1e-2
+
+
 def test_field_stride_selects_snapshots():
     """field_stride k keeps the fields after steps k-1, 2k-1, ... of the
     every-step output, in fp32 and fp64."""
