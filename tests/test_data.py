@@ -83,6 +83,28 @@ def test_load_all_vowels_order_labels_and_subsets(tmp_path):
     assert x1.shape == (1, 800) and y1.shape == (1, 3) and int(y1.argmax()) == 1
 
 
+def test_stratified_subset_picks_the_reference_files(tmp_path):
+    """max_samples subsets: the files are those the reference's split of the
+    per-gender lists themselves (vowels.py:75-81: waveforms, labels and paths
+    passed together, stratified on the labels) keeps, in the same order, for
+    several random states."""
+    from sklearn.model_selection import train_test_split
+    d = str(tmp_path)
+    names = [f"{g}{k:02d}{c}" for g in "wm" for c in ("ae", "ei", "iy") for k in range(1, 6)]
+    _write_set(d, names)
+    classes = ["ae", "ei", "iy"]
+    X, Y, F = load_all_vowels(classes, dir=d + "/")
+    xm, ym, fm = X[:15], [y.numpy() for y in Y[:15]], F[:15]
+    xw, yw, fw = X[15:], [y.numpy() for y in Y[15:]], F[15:]
+    for rs in (0, 1, 7, 123):
+        _, _, Fs = load_all_vowels(classes, dir=d + "/", max_samples=7, random_state=rs)
+        _, _, _, _, fm_k, _ = train_test_split(xm, ym, fm, train_size=3, test_size=3,
+                                               stratify=ym, shuffle=True, random_state=rs)
+        _, _, _, _, fw_k, _ = train_test_split(xw, yw, fw, train_size=4, test_size=3,
+                                               stratify=yw, shuffle=True, random_state=rs)
+        assert Fs == fm_k + fw_k, rs
+
+
 REF_VOWELS = "/root/reference/data/vowels"
 
 

@@ -303,7 +303,7 @@ def test_save_load_model_round_trip(tmp_path):
     save_model(m, "m1", str(tmp_path) + "/", cfg={"dtype": "float32"}, verbose=False)
     raw = torch.load(str(tmp_path / "m1.pt"), weights_only=True)
     assert set(raw) == {"model_geom_class_str", "model_state", "history", "history_geom_state",
-                        "cfg"}
+                        "cfg", "model_ext"}
     assert raw["model_geom_class_str"] == "WaveGeometryFreeForm"
     m2, hist, hgs, cfg = load_model(str(tmp_path / "m1.pt"))
     assert hist is None and cfg == {"dtype": "float32"} and len(hgs) == 1
@@ -313,6 +313,48 @@ def test_save_load_model_round_trip(tmp_path):
         [(p.x.item(), p.y.item()) for p in m.probes]
     assert [(q.x.item(), q.y.item()) for q in m2.sources] == \
         [(q.x.item(), q.y.item()) for q in m.sources]
+
+
+def test_save_load_nonlinear_round_trip(tmp_path):
+    """A nonlinear model (saturable damping, Kerr speed, clamp extension) and
+    a line source come back as saved: the cell's nonlinear buffers, its
+    loop configuration and the source's pixel list."""
+    import math
+    from wavetorch_amd import (WaveCell, WaveGeometryFreeForm, WaveIntensityProbe,
+                               WaveLineSource, WaveRNN)
+    from wavetorch_amd.io import load_model, save_model
+    geom = WaveGeometryFreeForm((40, 40), 1.0, c0=1.0, c1=0.5, abs_N=8, abs_sig=1.0,
+                                abs_p=2.0, rho="rand")
+    cell = WaveCell(0.6, geom, satdamp_b0=0.5, satdamp_uth=math.sqrt(0.1), c_nl=0.05,
+                    c_clamp=(0.4, 1.1))
+    m = WaveRNN(cell, [WaveLineSource(12, 10, 12, 29)],
+                [WaveIntensityProbe(30, 15), WaveIntensityProbe(30, 25)])
+    save_model(m, "nl", str(tmp_path) + "/", cfg={"dtype": "float32"}, verbose=False)
+    m2, _, _, _ = load_model(str(tmp_path / "nl.pt"))
+    for k in ("dt", "satdamp_b0", "satdamp_uth", "c_nl"):
+        assert torch.equal(getattr(m2.cell, k), getattr(m.cell, k)), k
+    assert m2.cell.c_clamp == (0.4, 1.1)
+    c1, c2 = m.loop_config(), m2.loop_config()
+    assert c2.nonlinear and c2 == c1
+    assert torch.equal(m2.sources[0].x, m.sources[0].x)
+    assert torch.equal(m2.sources[0].y, m.sources[0].y)
+    assert torch.equal(m2.cell.geom.rho.detach(), m.cell.geom.rho.detach())
+
+
+def test_load_reference_style_file(tmp_path):
+    """A file in the reference's layout (five entries, no model_ext, a
+    linear cell) loads: coordinates by index order, linear cell."""
+    from wavetorch_amd.io import load_model
+    m = _model()
+    rec = {"model_geom_class_str": "WaveGeometryFreeForm", "model_state": m.state_dict(),
+           "history": None,
+           "history_geom_state": [m.cell.geom.state_reconstruction_args()],
+           "cfg": {"dtype": "float32"}}
+    torch.save(rec, str(tmp_path / "ref.pt"))
+    m2, _, _, _ = load_model(str(tmp_path / "ref.pt"))
+    assert not m2.loop_config().nonlinear and m2.cell.c_clamp is None
+    assert [(p.x.item(), p.y.item()) for p in m2.probes] == [(40, 10), (40, 30)]
+    assert [(q.x.item(), q.y.item()) for q in m2.sources] == [(20, 25)]
 
 
 def test_product_path_has_no_cpu_fallback():

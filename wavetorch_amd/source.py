@@ -17,7 +17,20 @@ def bresenham_line(r0: int, c0: int, r1: int, c1: int):
     """Pixel rows/cols of the segment (r0,c0)-(r1,c1), one pixel per step along
     the major axis, endpoints included, in the order of the published
     algorithm of ``skimage.draw.line`` (used at source.py:31).  Axis-aligned
-    and 45-degree segments have a unique pixel set."""
+    and 45-degree segments have a unique pixel set.
+
+    This restates the Bresenham loop of scikit-image's ``skimage/draw/_draw.pyx``
+    (``_line``) step for step, so that the pixel order matches the one the
+    reference gets.  scikit-image is distributed under the BSD 3-Clause
+    license: Copyright (C) 2019, the scikit-image team.  All rights reserved.
+    Redistribution and use in source and binary forms, with or without
+    modification, are permitted provided that the copyright notice, this list
+    of conditions and the following disclaimer are retained; neither the name
+    of skimage nor the names of its contributors may be used to endorse or
+    promote products derived from this software without specific prior
+    written permission.  THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS
+    AND CONTRIBUTORS "AS IS" AND ANY EXPRESS OR IMPLIED WARRANTIES ARE
+    DISCLAIMED."""
     r, c = int(r0), int(c0)
     r1, c1 = int(r1), int(c1)
     steep = False
