@@ -50,6 +50,15 @@ int wt_abi_version(void);
  * NUL-terminated when len > 0) and returns its full length. */
 int wt_last_error(char *buf, size_t len);
 
+/* Sets flag[0] = 1 (device int32) when any of the n fp32 values at p is NaN
+ * or infinite, and leaves it unchanged otherwise: one kernel launch on
+ * `stream`, no host synchronisation, capturable into a graph.  The caller
+ * zeroes the flag.  No counterpart in the reference, where an overflowing
+ * field or gradient shows up only in the loss it prints (train.py:110-113);
+ * wavetorch_amd runs it on the loop's outputs and gradients when a model's
+ * check_finite is set. */
+int wt_check_finite(const float *p, int64_t n, int32_t *flag, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Material coefficients                                                     */
 /* ------------------------------------------------------------------------ */
@@ -152,8 +161,12 @@ int wt_plan_tuning(const void *plan, wt_tuning *out);
  *          probe amplitudes, and the record of the last checkpoint segment,
  *          which the forward writes so that the reverse sweep need not
  *          recompute that segment); needed only when with_grad.
- *   temp:  scratch of one call (state ping-pong, adjoint segment buffer,
- *          adjoint state, gradient partials). */
+ *   temp:  scratch of one call (state ping-pong, adjoint state, gradient
+ *          partials, and the segment record of the reverse sweep's
+ *          recomputes unless the plan keeps the last segment's record in
+ *          `saved`, which the sweep then reuses for them).
+ * wt_loop_fwd needs only the with_grad = 0 size of `temp`, with or without
+ * gradients. */
 int wt_workspace_bytes(const void *plan, int32_t with_grad, int64_t *saved,
                        int64_t *temp);
 
@@ -174,6 +187,8 @@ int wt_loop_fwd(void *plan, const float *b, const float *c, const float *rho,
 /* Reverse sweep (the autograd replay of cell.py:26-44 over rnn.py:50-70)
  * with checkpoint recompute every K steps (every segment but the last, whose
  * record the forward left in `saved`; with K >= T nothing is recomputed).
+ * Consumes `saved`: the recomputes overwrite the last segment's record, so
+ * each wt_loop_fwd(with_grad) is followed by at most one wt_loop_adj.
  *   ybar        (B,T,ntap) dJ/dy, or NULL
  *   fbar        (B,T,Nx,Ny) dJ/d(fields) (stride-1 fields only), or NULL
  *   grad_c      (Nx,Ny) dJ/dc_linear, summed over batch, or NULL

@@ -270,3 +270,35 @@ def test_single_segment_record_from_forward(cuda, nonlinear):
         assert next(iter(m._plans.values())).tuning["ck_every"] == K
     assert torch.equal(out[T][0], out[16][0])
     assert rel(out[T][1].numpy(), out[16][1].numpy()) < 1e-6
+
+
+def test_nonfinite_flags(cuda):
+    """check_finite: the on-device flags stay 0 for finite runs; a NaN input
+    sample sets the output flag, an infinite dJ/dy the gradient flag; the
+    training step stays free of host synchronisation with the checks on."""
+    from wavetorch_amd import workloads
+    from wavetorch_amd.utils import normalize_power
+    model, x, labels = workloads.propagate_100_b4(T=64)
+    model = model.to(cuda)
+    model.check_finite = True
+    x, labels = x.to(cuda), labels.to(cuda)
+    y = model(x)
+    torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels).backward()
+    assert model.nonfinite.tolist() == [0, 0]
+    xb = x.clone()
+    xb[1, 10] = float("nan")
+    model(xb)
+    assert model.nonfinite.tolist() == [1, 0]
+    model.nonfinite.zero_()
+    y = model(x)
+    g = torch.zeros_like(y)
+    g[0, 5, 1] = float("inf")
+    y.backward(g)
+    assert model.nonfinite.tolist() == [0, 1]
+    torch.cuda.synchronize()
+    torch.cuda.set_sync_debug_mode("error")
+    try:
+        y = model(x)
+        torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels).backward()
+    finally:
+        torch.cuda.set_sync_debug_mode(0)

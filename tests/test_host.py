@@ -156,6 +156,24 @@ def test_tuner_checkpoint_interval_options():
     assert t["ck_every"] == 1000 and autotune._from_tuning(t, with_k=True)["ck_every"] == 1000
 
 
+def test_checkpoint_interval_override():
+    """A checkpoint interval set on the model wins over a tuning's ck_every;
+    unset, the tuning's applies, else 64.  Tunings whose search chose K are
+    cached under K = "open"."""
+    from wavetorch_amd import autotune
+    m = _model()
+    assert m.checkpoint_every == 64
+    m.tuning = {"ck_every": 1000}
+    assert m.checkpoint_every == 1000
+    m.checkpoint_every = 32
+    assert m.checkpoint_every == 32
+    m.checkpoint_every = None
+    assert m.checkpoint_every == 1000
+    assert autotune.key(100, 100, 4, 1000, "open", "gfx950") == "gfx950:100x100:B4:T1000:Kopen"
+    t = autotune.cached("propagate_100_b4")
+    assert t is not None and t["ck_every"] == 1000
+
+
 def test_batch_candidates_checkpoint_interval():
     """Batch tuning candidates carry K = 64 and the smallest K >= 64 that both
     block depths divide (segments then end on full launches)."""

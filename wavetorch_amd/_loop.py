@@ -110,11 +110,23 @@ def _buf(nbytes, device):
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
 
 
+def _check_finite(t, flag, slot, stream):
+    if t is not None and flag is not None:
+        _lib.check(_lib.lib().wt_check_finite(t.data_ptr(), t.numel(), flag.data_ptr() + 4 * slot,
+                                              stream), "wt_check_finite")
+
+
 class WaveLoop(torch.autograd.Function):
-    """y = WaveRNN time loop (x, c, b, rho); backward = checkpointed adjoint."""
+    """y = WaveRNN time loop (x, c, b, rho); backward = checkpointed adjoint.
+
+    ``nonfinite``: None, or a device int32 tensor of two entries: entry 0 is
+    set to 1 when the forward's output holds a NaN or infinity, entry 1 when
+    a gradient the backward returns does (checked on the device with
+    ``wt_check_finite``; nothing waits for the host)."""
 
     @staticmethod
-    def forward(ctx, x, c, b, rho, plan: Plan, output_fields: bool, field_stride: int):
+    def forward(ctx, x, c, b, rho, plan: Plan, output_fields: bool, field_stride: int,
+                nonfinite=None):
         dev = x.device
         x = x.contiguous()
         c = c.contiguous()
@@ -132,13 +144,15 @@ class WaveLoop(torch.autograd.Function):
         else:
             y = torch.empty((B, T, ntap), dtype=torch.float32, device=dev)
         saved = _buf(plan.saved_bytes, dev) if with_grad else None
-        temp = _buf(plan.temp_bytes if with_grad else plan.temp_bytes_nograd, dev)
+        temp = _buf(plan.temp_bytes_nograd, dev)  # the forward's scratch, with or without grad
         s = _lib.stream_ptr(dev)
         _lib.check(_lib.lib().wt_loop_fwd(
             plan.handle, b.data_ptr(), c.data_ptr(), _lib.ptr(rho_c), x.data_ptr(), _lib.ptr(y),
             _lib.ptr(fields), int(field_stride), int(with_grad), _lib.ptr(saved),
             temp.data_ptr(), s), "wt_loop_fwd")
         del temp
+        _check_finite(fields if output_fields else y, nonfinite, 0, s)
+        ctx.nonfinite = nonfinite
         if with_grad:
             ctx.save_for_backward(x, c, b, rho_c if rho_c is not None else x.new_empty(0))
             ctx.ws = saved
@@ -173,4 +187,6 @@ class WaveLoop(torch.autograd.Function):
             ctx.ws.data_ptr(), temp.data_ptr(), _lib.ptr(gc), _lib.ptr(gb), _lib.ptr(grho),
             _lib.ptr(gx), s), "wt_loop_adj")
         ctx.ws = None
-        return gx, gc, gb, grho, None, None, None
+        for gt in (gx, gc, gb, grho):
+            _check_finite(gt, ctx.nonfinite, 1, s)
+        return gx, gc, gb, grho, None, None, None, None

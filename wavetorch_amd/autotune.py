@@ -49,6 +49,8 @@ def _store(d):
 
 
 def key(Nx, Ny, B, T, K, arch=None, forward_only=False):
+    """Cache key of a problem shape; K is the checkpoint interval, or "open"
+    for tunings whose search chose it (they carry ``ck_every``)."""
     return f"{arch or _arch()}:{Nx}x{Ny}:B{B}:T{T}:K{K}" + (":fwd" if forward_only else "")
 
 
@@ -61,7 +63,7 @@ def lookup(Nx, Ny, B, T, K=64, arch="gfx950", forward_only=False):
 def cached(workload: str):
     """Tuning for a named workload of wavetorch_amd.workloads, or None."""
     shapes = {"train_150_b64": (150, 150, 64, 3000, 64),
-              "propagate_100_b4": (100, 100, 4, 1000, 64),
+              "propagate_100_b4": (100, 100, 4, 1000, "open"),
               "nonlinear_150_b32": (150, 150, 32, 3000, 64),
               "scatter_1024": (1024, 1024, 1, 4000, 64, "gfx950", True)}
     if workload not in shapes:
@@ -140,13 +142,19 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
     option of one field at a time (the rest fixed), keeps any improvement and
     repeats for `rounds` passes or until `budget_s`; caches the winner under
     key(Nx, Ny, B, T, K) and returns it.  With `tune_k` the checkpoint
-    interval is searched too (k_options; the winner carries ``ck_every``) --
-    for configurations that leave K open: more memory for fewer recomputed
-    segments."""
+    interval is searched too, from the model's K (k_options; the winner
+    carries ``ck_every`` and is cached under K = "open") -- for
+    configurations that leave K open: more memory for fewer recomputed
+    segments.  A checkpoint interval set on the model is set aside during
+    that search and restored after it (it then still overrides the
+    tuning's)."""
     from .utils import normalize_power
     B, T = x.shape
     Nx, Ny = model.cell.geom.domain_shape
     K = int(model.checkpoint_every)
+    user_k = model._checkpoint_every
+    if tune_k:
+        model.checkpoint_every = None
     t_start = time.perf_counter()
 
     def step():
@@ -184,8 +192,9 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
         return plan.tuning, t
 
     model.tuning = None
-    base, best_t = measure({})
+    base, best_t = measure({"ck_every": K} if tune_k else {})
     if base is None:
+        model.checkpoint_every = user_k
         return None
     state = _from_tuning(base, with_k=tune_k)
     tried = {tuple(sorted(state.items()))}
@@ -228,10 +237,11 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
             break
     best = _to_tuning(state)
     model.tuning = best
+    model.checkpoint_every = user_k
     with _lock:
         d = _load()
-        d[key(Nx, Ny, B, T, K, forward_only=forward_only)] = {"tuning": best,
-                                                              "ms": best_t * 1e3}
+        d[key(Nx, Ny, B, T, "open" if tune_k else K, forward_only=forward_only)] = {
+            "tuning": best, "ms": best_t * 1e3}
         _store(d)
     return best
 

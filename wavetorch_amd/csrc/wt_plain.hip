@@ -436,6 +436,17 @@ __global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+// flag[0] = 1 when any of the n values is NaN or infinite (exponent all
+// ones); left unchanged otherwise.  Every writer stores the same value, so no
+// atomic is needed.
+__global__ void k_nonfinite(const float *__restrict__ p, int64_t n, int32_t *__restrict__ flag) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  bool bad = false;
+  for (; i < n; i += stride) bad |= (__float_as_uint(p[i]) & 0x7f800000u) == 0x7f800000u;
+  if (bad) flag[0] = 1;
+}
+
 // ------------------------------------------------------------- launchers
 
 static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -571,6 +582,14 @@ hipError_t launch_finalize_gx(const float *gsrc, int B, int T, int nslot, const 
   int64_t n = (int64_t)B * T;
   hipLaunchKernelGGL(k_finalize_gx, dim3(nblk(n, 256)), dim3(256), 0, s, gsrc, B, T, nslot,
                      mult, gx);
+  return hipGetLastError();
+}
+
+hipError_t launch_nonfinite(const float *p, int64_t n, int32_t *flag, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = nblk(n, 256);
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)blocks), dim3(256), 0, s, p, n, flag);
   return hipGetLastError();
 }
 

@@ -67,6 +67,7 @@ hipError_t launch_finalize_gc(const float *gacc, int B, int64_t F, const float *
 hipError_t launch_finalize_gx(const float *gsrc, int B, int T, int nslot, const int32_t *mult,
                               float *gx, hipStream_t s);
 hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t s);
+hipError_t launch_nonfinite(const float *p, int64_t n, int32_t *flag, hipStream_t s);
 hipError_t launch_fwd_tiled(int nt, int cpt, bool unit_h, int nl, bool rec, const FwdArgs &a,
                             int LWmax, hipStream_t s);
 hipError_t launch_adj_tiled(int nt, int cpt, bool unit_h, const AdjArgs &a, int LWmax,
@@ -380,7 +381,10 @@ static int plan_layout(Plan &p) {
   p.t_lam = align_up((nlc ? 12 : 4) * BF);
   p.t_wl = align_up(2 * BF);     // plain adjoint w*lambda
   p.t_gacc = align_up((nlc ? 3 : 1) * BF);
-  p.t_seg = align_up((nlc ? 2 : 1) * segcells * sizeof(float));
+  // segment record of the reverse sweep's recomputes: when the forward keeps
+  // the last segment's record in `saved`, the sweep consumes that record
+  // first and then recomputes every earlier segment into the same buffer
+  p.t_seg = p.saved_rec ? 0 : align_up((nlc ? 2 : 1) * segcells * sizeof(float));
   p.t_gsrc = align_up((size_t)p.B * p.T * std::max(p.nslot, 1) * sizeof(float));
   p.t_extra = align_up(ncoef * F * sizeof(float));  // coefficients when not saving
   p.temp_bytes = p.t_state + p.t_lam + p.t_wl + p.t_gacc + p.t_seg + p.t_gsrc + p.t_extra;
@@ -417,6 +421,15 @@ static void scalars(float dt, float h, float &dtm1, float &dtm2, float &s2, floa
   s2 = 2.0f * dtm2;
   t1 = 1.0f / (h * h);
   t4 = -4.0f * t1;
+}
+
+int wt_check_finite(const float *p, int64_t n, int32_t *flag, void *stream) {
+  if (!p || !flag || n < 0) {
+    set_error("wt_check_finite: invalid argument");
+    return WT_EINVAL;
+  }
+  WT_HIP(launch_nonfinite(p, n, flag, (hipStream_t)stream));
+  return WT_OK;
 }
 
 int wt_coef_linear(const float *b, const float *c, int64_t n, float dt, float *A, float *Bm,
@@ -921,7 +934,7 @@ Ws layout(const Plan &p, void *saved, void *temp, bool with_grad) {
     t += p.t_wl;
     w.gacc = (float *)t;
     t += p.t_gacc;
-    w.seg = (float *)t;
+    w.seg = w.rec ? w.rec : (float *)t;
     t += p.t_seg;
     w.gsrc = (float *)t;
     t += p.t_gsrc;
@@ -962,7 +975,7 @@ FwdArgs base_fwd(const Plan &p, const Ws &w, const float *b, const float *c, con
 // when given.  Plain: one launch per step rotating through tmp[0..2]; the
 // final state is copied to (out1, out2) when given.
 int run_forward(const Plan &p, FwdArgs a, int n0, int n1, const float *in1, const float *in2,
-                float *out1, float *out2, float *tmp[4], bool rec, hipStream_t s) {
+                float *out1, float *out2, float *tmp[4], bool rec, int prof_cls, hipStream_t s) {
   const int64_t BF = (int64_t)p.B * p.F;
   if (p.tiled) {
     const float *i1 = in1, *i2 = in2;
@@ -975,7 +988,7 @@ int run_forward(const Plan &p, FwdArgs a, int n0, int n1, const float *in1, cons
       float *o2 = (last && out2) ? out2 : tmp[2 * pp + 1];
       a.in1 = i1; a.in2 = i2; a.out1 = o1; a.out2 = o2;
       a.n0 = n; a.nsteps = k;
-      ProfScope ps(rec ? 1 : 0, s, (int64_t)p.B * p.Nx * p.Ny * k);
+      ProfScope ps(prof_cls, s, (int64_t)p.B * p.Nx * p.Ny * k);
       if (p.tun.kind == 1)
         WT_HIP(launch_fwd_tiled(p.tun.nt, p.tun.cpt, p.unit_h, p.nl, rec, a, p.LWmax, s));
       else
@@ -993,7 +1006,7 @@ int run_forward(const Plan &p, FwdArgs a, int n0, int n1, const float *in1, cons
     WT_HIP(hipMemcpyAsync(r2, in2, BF * sizeof(float), hipMemcpyDeviceToDevice, s));
     for (int n = n0; n < n1; ++n) {
       a.in1 = r1; a.in2 = r2; a.out1 = r3;
-      ProfScope ps(rec ? 1 : 0, s, (int64_t)p.B * p.Nx * p.Ny);
+      ProfScope ps(prof_cls, s, (int64_t)p.B * p.Nx * p.Ny);
       WT_HIP(launch_fwd_plain(p.unit_h, p.nl, rec, a, p.d_spmap, n, s));
       float *t = r2; r2 = r1; r1 = r3; r3 = t;
     }
@@ -1082,7 +1095,7 @@ int adj_nonlinear(const Plan &p, const float *b, const float *c, const float *rh
       // the last segment's record came with the forward
       const bool from_fwd = w.rec && sg == p.nseg - 1;
       if (!from_fwd) {
-        int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+        int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, 1, s);
         if (rc) return rc;
       }
       aa.seg_lap = from_fwd ? w.rec : fa.rec_lap;
@@ -1113,7 +1126,7 @@ int adj_nonlinear(const Plan &p, const float *b, const float *c, const float *rh
     float *tmp[4] = {st[0], st[1], st[2], st[3]};
     const bool from_fwd = w.rec && sg == p.nseg - 1;
     if (!from_fwd) {
-      int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+      int rc = run_forward(p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, 1, s);
       if (rc) return rc;
     }
     aa.seg_lap = from_fwd ? w.rec : fa.rec_lap;
@@ -1187,9 +1200,9 @@ int wt_loop_fwd(void *plan, const float *b, const float *c, const float *rho, co
       ar.rec_u = p->nl ? w.rec + (size_t)p->B * p->K * p->Nx * (p->Ny + 1) : nullptr;
       ar.rec_K = p->K;
       ar.rec_n0 = n0;
-      rc = run_forward(*p, ar, n0, n1, cur1, cur2, o1, o1 + BF, tmp, true, s);
+      rc = run_forward(*p, ar, n0, n1, cur1, cur2, o1, o1 + BF, tmp, true, 0, s);
     } else {
-      rc = run_forward(*p, a, n0, n1, cur1, cur2, o1, o1 + BF, tmp, false, s);
+      rc = run_forward(*p, a, n0, n1, cur1, cur2, o1, o1 + BF, tmp, false, 0, s);
     }
     if (rc) return rc;
     cur1 = o1;
@@ -1251,11 +1264,11 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho, co
     if (!from_fwd) {
       int rc;
       if (use_tiled) {
-        rc = run_forward(*p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+        rc = run_forward(*p, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, 1, s);
       } else {
         Plan q = *p;
         q.tiled = false;
-        rc = run_forward(q, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, s);
+        rc = run_forward(q, fa, n0, n1, ck, ck + BF, nullptr, nullptr, tmp, true, 1, s);
       }
       if (rc) return rc;
     }

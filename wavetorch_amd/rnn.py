@@ -31,12 +31,47 @@ class WaveRNN(torch.nn.Module):
         # kernel tuning (wt_tuning fields); None = library heuristics.  Set by
         # wavetorch_amd.autotune.tune(model, ...) or by hand.
         self.tuning = None
-        # checkpoint interval K of the adjoint (steps)
-        self.checkpoint_every = 64
+        # checkpoint interval K of the adjoint (steps); None = the tuning's
+        # ck_every, else 64.  A value set here overrides the tuning's.
+        self._checkpoint_every = None
         # extension: keep only every field_stride-th field when output_fields
         self.field_stride = 1
+        # extension: on-device non-finite checks of the loop's output and of
+        # the gradients it returns (see nonfinite)
+        self.check_finite = False
+        self._nonfinite = None
         self._plans = {}
         self._cfg_cache = None
+
+    @property
+    def checkpoint_every(self) -> int:
+        """Checkpoint interval K of the adjoint: the value set on the model,
+        else the tuning's ``ck_every``, else 64."""
+        if self._checkpoint_every is not None:
+            return self._checkpoint_every
+        return int((self.tuning or {}).get("ck_every") or 64)
+
+    @checkpoint_every.setter
+    def checkpoint_every(self, k):
+        self._checkpoint_every = None if k is None else int(k)
+
+    @property
+    def nonfinite(self):
+        """With ``check_finite`` set: a device int32 tensor of two flags,
+        [output, gradient], set to 1 by any forward whose output (probe
+        series or fields) or any backward whose returned gradients hold a
+        NaN or an infinity since the last ``zero_()`` of this tensor.  The
+        checks run on the device and never synchronise with the host; read
+        the flags when convenient.  None until a checked forward has run."""
+        return self._nonfinite
+
+    def _nonfinite_flags(self, device):
+        if not self.check_finite:
+            return None
+        f = self._nonfinite
+        if f is None or f.device != device:
+            f = self._nonfinite = torch.zeros(2, dtype=torch.int32, device=device)
+        return f
 
     # ------------------------------------------------------------ config
     def _buffers_key(self):
@@ -87,7 +122,7 @@ class WaveRNN(torch.nn.Module):
     def plan(self, B: int, T: int, device) -> Plan:
         cfg = self.loop_config()
         tun = dict(self.tuning) if self.tuning else {}
-        tun.setdefault("ck_every", int(self.checkpoint_every))
+        tun["ck_every"] = self.checkpoint_every
         tkey = tuple(sorted(tun.items()))
         key = (cfg, B, T, str(device), tkey)
         p = self._plans.get(key)
@@ -121,7 +156,8 @@ class WaveRNN(torch.nn.Module):
         b = geom.b
         rho = geom.rho if self.loop_config().nonlinear else None
         plan = self.plan(B, T, x.device)
-        out = WaveLoop.apply(x, c, b, rho, plan, fields_mode, stride)
+        out = WaveLoop.apply(x, c, b, rho, plan, fields_mode, stride,
+                             self._nonfinite_flags(x.device))
         if fields_mode:
             return out
         shp = self.probes[0].out_shape()  # from tensor shapes: no device read
