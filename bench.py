@@ -10,13 +10,22 @@ are resident on the GPU before the timed region.
 
 Prints ONE JSON line:
   value          Gcell-steps/s = B*Nx*Ny*T / step time / 1e9
-  roofline       the dominant kernel (by summed time) at 12 B (forward) or
-                 16 B (adjoint) of algorithmic HBM traffic per cell-step,
-                 timed with HIP events on its launch stream over a second
-                 timed region of the same length, against MI355X's 8 TB/s;
-                 `traffic` from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-                 (--calib-copy / --summarize-pmc) when one matches the plan
-  step_roofline  the whole step at its algorithmic bytes per cell-step
+  roofline       the dominant kernel (by summed time), its average launch
+                 time from HIP events on its launch stream (a second timed
+                 region of the same length), against the unit that binds it:
+                 HBM (the bytes rocprofv3's FETCH_SIZE + WRITE_SIZE count per
+                 launch, calibrated on a device copy, against 8 TB/s) or the
+                 vector ALUs (SQ_INSTS_VALU per launch x 64 lanes against one
+                 fp32 lane-operation per lane per clock: 256 CUs x 64 lanes
+                 x 2.4 GHz); `frac` is the larger of the two.  The counters
+                 come from profiles/traffic_<workload>.json (--calib-copy /
+                 --summarize-pmc) when it was taken on the same plan.
+                 `effective_bw` keeps the algorithmic figure (12 B forward,
+                 16 B adjoint per cell-step x cell-steps per launch / time),
+                 which temporal blocking lets exceed the HBM peak: it is not
+                 a bound.
+  step_roofline  the whole step at its algorithmic bytes per cell-step (the
+                 same caveat)
   cpu_baseline   the torch-CPU restatement of the reference (oracle/ref_torch)
                  on a bounded sample of the same workload on this host's cores
   plain          the same step with the one-launch-per-step kernels
@@ -36,6 +45,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12
+# non-packed fp32 VALU issue: 256 CUs x 4 SIMDs x 16 lanes per clock at the
+# 2.4 GHz peak engine clock (the 157.3 TFLOP/s figure counts packed FMAs)
+PEAK_VALU_LANE_OPS = 256 * 64 * 2.4e9
 BYTES = {0: 12.0, 1: 12.0, 2: 16.0}  # algorithmic bytes per cell-step per kernel class
 
 def traffic_file(workload):
@@ -193,25 +205,78 @@ def time_steps(model, x, labels, steps, warmup, profile=False, fused_loss=False)
     return dt, float(loss.item()), prof
 
 
-def traffic(workload, kname, tun):
-    """Measured HBM bytes per launch of `kname` from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary written by --summarize-pmc, when it was taken on the
-    same kernel and tuning; else None."""
+def kernel_counters(workload, kname, tun):
+    """Per-launch counters of `kname` from the rocprofv3 summary written by
+    --summarize-pmc (hbm_bytes_per_launch, and SQ_* counters when those runs
+    were taken), when it was taken on the same kernel and tuning; else {}."""
     try:
         with open(traffic_file(workload)) as f:
             d = json.load(f)
     except (OSError, ValueError):
-        return None
+        return {}
     k = d.get("kernels", {}).get(kname)
     if k is None or d.get("tuning") != tun:
-        return None
-    return k["hbm_bytes_per_launch"]
+        return {}
+    return k
 
 
-def summarize_pmc(fetch_dir, write_dir, calib_bytes):
+def traffic(workload, kname, tun):
+    """Measured HBM bytes per launch of `kname` (FETCH_SIZE + WRITE_SIZE,
+    calibrated), or None."""
+    return kernel_counters(workload, kname, tun).get("hbm_bytes_per_launch")
+
+
+def roofline(workload, kname, tun, cls, per_launch, avg_s, launches, extra=None):
+    """The roofline object of the dominant kernel: the fraction of HBM and of
+    VALU issue its measured counters reach at its measured launch time,
+    `frac` the larger (the binding unit), and the algorithmic-equivalent
+    bandwidth under its own key."""
+    k = kernel_counters(workload, kname, tun)
+    out = {"kernel": kname, "avg_launch_us": round(avg_s * 1e6, 2), "launches": launches,
+           "cell_steps_per_launch": per_launch}
+    units = {}
+    if k.get("hbm_bytes_per_launch"):
+        bw = k["hbm_bytes_per_launch"] / avg_s
+        units["hbm"] = {"achieved": round(bw / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                        "frac": round(bw / PEAK_HBM, 4),
+                        "bytes_per_launch": k["hbm_bytes_per_launch"]}
+    if k.get("SQ_INSTS_VALU"):
+        ops = k["SQ_INSTS_VALU"] * 64 / avg_s
+        units["valu"] = {"achieved": round(ops / 1e12, 2),
+                         "peak": round(PEAK_VALU_LANE_OPS / 1e12, 2),
+                         "unit": "Tlane-op/s", "frac": round(ops / PEAK_VALU_LANE_OPS, 4),
+                         "insts_per_launch": k["SQ_INSTS_VALU"]}
+    for key_ in ("SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_WAIT_INST_ANY", "SQ_BUSY_CYCLES",
+                 "SQ_WAVE_CYCLES", "SQ_ACTIVE_INST_VALU", "SQ_INSTS_SALU"):
+        if key_ in k:
+            out.setdefault("counters_per_launch", {})[key_] = k[key_]
+    if units:
+        bound = max(units, key=lambda u: units[u]["frac"])
+        b = units[bound]
+        out.update({"bound": bound, "achieved": b["achieved"], "peak": b["peak"],
+                    "unit": b["unit"], "frac": b["frac"]})
+        out.update(units)
+    else:  # no counter summary for this plan: no measured fraction
+        out.update({"bound": "hbm", "achieved": None, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                    "frac": None})
+    out["traffic"] = k.get("hbm_bytes_per_launch")
+    eff = BYTES[cls] * per_launch / avg_s
+    out["effective_bw"] = {"basis": f"{BYTES[cls]:g} algorithmic B per cell-step x cell-steps "
+                                    "per launch / launch time (not a bound)",
+                           "achieved": round(eff / 1e9, 1), "unit": "GB/s",
+                           "frac_of_hbm_peak": round(eff / PEAK_HBM, 4)}
+    if extra:
+        out.update(extra)
+    return out
+
+
+def summarize_pmc(fetch_dir, write_dir, calib_bytes, extra_dirs=()):
     """Per-kernel average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two
     rocprofv3 --pmc runs, each scaled by what the counter reads for the
-    bench's own device copy of `calib_bytes` bytes; writes profiles/traffic_<workload>.json."""
+    bench's own device copy of `calib_bytes` bytes, plus the per-dispatch
+    average of every counter found in `extra_dirs` (further --pmc runs, e.g.
+    SQ_INSTS_VALU, SQ_INSTS_LDS, SQ_LDS_BANK_CONFLICT, SQ_WAIT_INST_ANY);
+    writes profiles/traffic_<workload>.json."""
     import collections
     import csv
     import glob
@@ -223,6 +288,17 @@ def summarize_pmc(fetch_dir, write_dir, calib_bytes):
                 if r["Counter_Name"] == counter:
                     agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
         return agg
+
+    def read_all(d):
+        agg = collections.defaultdict(lambda: collections.defaultdict(list))
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                agg[r["Kernel_Name"]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        return agg
+
+    def short(k):
+        name = k.split("(")[0].strip()
+        return name[5:] if name.startswith("void ") else name
 
     fa, wa = read(fetch_dir, "FETCH_SIZE"), read(write_dir, "WRITE_SIZE")
     fe = {k: sum(v) / len(v) for k, v in fa.items()}
@@ -244,8 +320,7 @@ def summarize_pmc(fetch_dir, write_dir, calib_bytes):
     for k in sorted(fe, key=lambda k: -fe[k]):
         if k == ck or k not in wr:
             continue
-        name = k.split("(")[0].strip()
-        name = name[5:] if name.startswith("void ") else name
+        name = short(k)
         fb, wb = fe[k] * 1024.0 * f_scale, wr[k] * 1024.0 * w_scale
         out["kernels"][name] = {"dispatches": len(fa[k]), "fetch_kib": fe[k], "write_kib": wr[k],
                                 "fetch_bytes": fb, "write_bytes": wb,
@@ -253,6 +328,15 @@ def summarize_pmc(fetch_dir, write_dir, calib_bytes):
         lines.append(f"{name}: {len(fa[k])} dispatches, FETCH {fe[k]:.1f} KiB, WRITE "
                      f"{wr[k]:.1f} KiB -> fetch {fb / 1e6:.3f} MB  write {wb / 1e6:.3f} MB "
                      f"per launch")
+    for d in extra_dirs:
+        for k, cnt in read_all(d).items():
+            name = short(k)
+            if name not in out["kernels"]:
+                continue
+            for cn, vals in cnt.items():
+                out["kernels"][name][cn] = sum(vals) / len(vals)
+            lines.append(f"{name}: " + ", ".join(f"{cn} {sum(v) / len(v):.4g}"
+                                                 for cn, v in sorted(cnt.items())))
     path = traffic_file(meta.get("workload", "train_150_b64"))
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
@@ -427,7 +511,6 @@ def run_batch(args, dev):
     msc, nl, work = prof[cls]
     avg_s = msc / max(nl, 1) / 1e3
     per_launch = work / max(nl, 1)
-    achieved = BYTES[cls] * per_launch / avg_s
     t0g = bt.tuning[0]
     names = [f"wt::k_fwd_cs_batch<{t0g['nt']}, {t0g['cpt']}, 0, false>",
              f"wt::k_fwd_cs_batch<{t0g['nt']}, {t0g['cpt']}, 0, true>",
@@ -447,16 +530,12 @@ def run_batch(args, dev):
                                "strips": bt.tuning[g]["strips"], "tcols": bt.tuning[g]["tcols"]}
                               for g, idx in enumerate(bt.groups)],
                    "cell_steps": cells, "tuning": bt.tuning[0]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
-                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM, 4),
-                     "traffic": traffic(args.workload, names[cls], bt.tuning[0]),
-                     "kernel": names[cls], "bytes_per_cell_step": BYTES[cls],
-                     "cell_steps_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
-                     "launches": nl, "eager_ms": round(dt_eager * 1e3, 3)},
+        "roofline": roofline(args.workload, names[cls], bt.tuning[0], cls, per_launch, avg_s,
+                             nl, {"eager_ms": round(dt_eager * 1e3, 3)}),
         "step_roofline": {"bytes_per_cell_step": wl["bytes"],
+                          "basis": "algorithmic bytes (not a bound)",
                           "achieved": round(wl["bytes"] * cells / dt / 1e9, 1), "unit": "GB/s",
-                          "frac": round(wl["bytes"] * cells / dt / PEAK_HBM, 4)},
+                          "frac_of_hbm_peak": round(wl["bytes"] * cells / dt / PEAK_HBM, 4)},
     }
     if args.meta_out:
         with open(args.meta_out, "w") as f:
@@ -550,11 +629,17 @@ def main():
                          "known-size copy that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--meta-out", type=str, default=None,
                     help="write the resolved tuning to this JSON file")
-    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "BYTES"),
-                    help="summarise two rocprofv3 --pmc runs into the traffic file and exit")
+    ap.add_argument("--summarize-pmc", nargs="+", metavar="DIR",
+                    help="FETCH_DIR WRITE_DIR BYTES [COUNTER_DIR ...]: summarise the "
+                         "FETCH_SIZE and WRITE_SIZE rocprofv3 --pmc runs (calibrated on the "
+                         "BYTES-byte copy) and any further --pmc runs into the traffic file "
+                         "and exit")
     args = ap.parse_args()
     if args.summarize_pmc:
-        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], int(args.summarize_pmc[2]))
+        if len(args.summarize_pmc) < 3:
+            ap.error("--summarize-pmc needs FETCH_DIR WRITE_DIR BYTES")
+        summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], int(args.summarize_pmc[2]),
+                      args.summarize_pmc[3:])
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -590,7 +675,6 @@ def main():
     ms, nl, work = prof[cls]
     avg_s = ms / max(nl, 1) / 1e3
     per_launch = work / max(nl, 1)
-    achieved = BYTES[cls] * per_launch / avg_s
     nonlin = plan.cfg.nonlinear
     for c, (m_, n_, w_) in prof.items():
         log(f"[bench] class {c} {class_name(c, plan.tuning, nonlin)}: {n_} launches, "
@@ -611,23 +695,30 @@ def main():
         "data": "synthetic (seeded inputs and rho, see wavetorch_amd/workloads.py)",
         "config": {"workload": f"{args.workload}: {wl['desc']}", "B": B, "Nx": Nx, "Ny": Ny,
                    "T": T, "tuning": plan.tuning},
-        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
-                     "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": round(achieved / PEAK_HBM, 4),
-                     "traffic": traffic(args.workload, kname, plan.tuning),
-                     "kernel": kname, "bytes_per_cell_step": BYTES[cls],
-                     "cell_steps_per_launch": per_launch, "avg_launch_us": round(avg_s * 1e6, 2),
-                     "launches": nl, "events_ms_per_step": round(prof_step_s * 1e3, 3),
-                     "kernel_ms_per_step": {class_name(c, plan.tuning, nonlin):
-                                            round(prof[c][0] / args.steps, 3)
-                                            for c in prof if prof[c][1]}},
+        "roofline": roofline(args.workload, kname, plan.tuning, cls, per_launch, avg_s, nl,
+                             {"events_ms_per_step": round(prof_step_s * 1e3, 3),
+                              "kernel_ms_per_step": {class_name(c, plan.tuning, nonlin):
+                                                     round(prof[c][0] / args.steps, 3)
+                                                     for c in prof if prof[c][1]}}),
         # the whole step at its algorithmic bytes per cell-step (28 B = 12 forward
         # + 16 adjoint for a training step, 12 B forward only)
         "step_roofline": {"bytes_per_cell_step": wl["bytes"],
+                          "basis": "algorithmic bytes (not a bound)",
                           "achieved": round(wl["bytes"] * cells / dt / 1e9, 1), "unit": "GB/s",
-                          "frac": round(wl["bytes"] * cells / dt / PEAK_HBM, 4)},
+                          "frac_of_hbm_peak": round(wl["bytes"] * cells / dt / PEAK_HBM, 4)},
     }
     if labels is not None:
         out["loss_head"] = args.loss
+        # the timed step's loss, then one more step with the on-device
+        # non-finite checks of the loop's output and gradients
+        out["loss_finite"] = bool(np.isfinite(loss))
+        model.check_finite = True
+        train_step(model, x, labels, torch.optim.Adam(model.parameters(), lr=1e-3), fused)
+        flags = model.nonfinite.tolist()
+        model.check_finite = False
+        out["output_finite"], out["grad_finite"] = flags[0] == 0, flags[1] == 0
+        if not (out["loss_finite"] and flags == [0, 0]):
+            log(f"[bench] non-finite values: loss {loss}, flags (output, gradient) {flags}")
     if args.meta_out:
         with open(args.meta_out, "w") as f:
             json.dump({"tuning": plan.tuning, "workload": args.workload}, f)
@@ -648,7 +739,7 @@ def main():
     del model
     if not args.no_plain:
         pm, px, pl = build(dev, {"plain": 1}, args.workload)
-        pdt, _, _ = time_steps(pm, px, pl, 1, 1)
+        pdt, _, _ = time_steps(pm, px, pl, max(5, args.steps), 2)
         log(f"[bench] plain: {pdt * 1e3:.2f} ms/step")
         out["plain_ms_per_step"] = round(pdt * 1e3, 3)
         out["speedup_vs_plain"] = round(pdt / dt, 3)

@@ -55,6 +55,33 @@ def test_summarize_pmc_calibrates_on_largest_copy(tmp_path, monkeypatch):
     assert bench.traffic("unit_test", "wt::k_adj_cs<1024, 8>", {"a": 2}) is None
 
 
+def test_roofline_takes_the_binding_unit(tmp_path, monkeypatch):
+    """roofline(): HBM fraction from the counted bytes, VALU fraction from
+    SQ_INSTS_VALU x 64 lanes against 256 CUs x 64 lanes x 2.4 GHz; `frac`
+    and `bound` are the larger one; the algorithmic figure sits apart under
+    effective_bw; with no counters for the plan there is no fraction."""
+    _write_counters(tmp_path / "f", "FETCH_SIZE", [("copy", 1024.0), ("void wt::k(x)", 400.0)])
+    _write_counters(tmp_path / "w", "WRITE_SIZE", [("copy", 1024.0), ("void wt::k(x)", 100.0)])
+    _write_counters(tmp_path / "v", "SQ_INSTS_VALU", [("void wt::k(x)", 3.0e6),
+                                                      ("void wt::k(x)", 5.0e6)])
+    (tmp_path / "f" / "bench_meta.json").write_text(json.dumps({"tuning": {"a": 1},
+                                                                "workload": "unit_test"}))
+    monkeypatch.setattr(bench, "traffic_file", lambda w: str(tmp_path / f"traffic_{w}.json"))
+    bench.summarize_pmc(str(tmp_path / "f"), str(tmp_path / "w"), 2 ** 20, [str(tmp_path / "v")])
+    avg_s = 40e-6
+    r = bench.roofline("unit_test", "wt::k", {"a": 1}, 2, 1.0e7, avg_s, 10)
+    hbm = (400.0 + 100.0) * 1024 / avg_s / bench.PEAK_HBM
+    valu = 4.0e6 * 64 / avg_s / (256 * 64 * 2.4e9)
+    assert r["hbm"]["frac"] == pytest.approx(hbm, rel=1e-3)
+    assert r["valu"]["frac"] == pytest.approx(valu, rel=1e-3)
+    assert r["bound"] == ("valu" if valu > hbm else "hbm")
+    assert r["frac"] == pytest.approx(max(hbm, valu), rel=1e-3) and r["frac"] <= 1.0
+    assert r["effective_bw"]["frac_of_hbm_peak"] == pytest.approx(16 * 1e7 / avg_s / 8e12,
+                                                                  rel=1e-3)
+    none = bench.roofline("unit_test", "wt::k", {"a": 2}, 2, 1.0e7, avg_s, 10)
+    assert none["frac"] is None and none["traffic"] is None
+
+
 def test_multi_gpu_request_exits_with_message():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "4"],
                        capture_output=True, text=True)
