@@ -34,6 +34,29 @@ FLAGS = [
 ]
 
 
+# Host-side AddressSanitizer build (plan, tiling and table construction,
+# argument checks): the sanitizer applies to the host pass only, device code
+# is compiled as in the product build.  The library's __asan_* references are
+# resolved at load time from the clang runtime, which the loading process
+# preloads (tests/test_host_asan.py).
+ASAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer",
+              "-Xarch_host", "-g"]
+ASAN_OUT = os.path.join(OUT, "asan")
+
+
+def asan_runtime() -> str:
+    """Path of clang's shared AddressSanitizer runtime next to hipcc."""
+    import glob
+    cands = sorted(glob.glob(os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "lib",
+                                          "clang", "*", "lib", "linux",
+                                          "libclang_rt.asan-x86_64.so")))
+    if not cands:
+        cands = sorted(glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so"))
+    if not cands:
+        raise FileNotFoundError("libclang_rt.asan-x86_64.so not found next to hipcc")
+    return os.path.realpath(cands[-1])
+
+
 def _compile(src: str, obj: str, verbose: bool, extra=()) -> None:
     cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(HERE, src), "-o", obj]
     if verbose:
@@ -71,5 +94,13 @@ def build(verbose: bool = False, jobs: int = 4, out_dir: str | None = None,
     return lib
 
 
+def build_asan(verbose: bool = False, jobs: int = 4) -> str:
+    """The host-AddressSanitizer variant of the library, in ``_lib/asan``."""
+    return build(verbose=verbose, jobs=jobs, out_dir=ASAN_OUT, extra_flags=ASAN_FLAGS)
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    if "--asan" in sys.argv:
+        print(build_asan(verbose="-v" in sys.argv))
+    else:
+        print(build(verbose="-v" in sys.argv))
