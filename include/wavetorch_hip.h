@@ -19,8 +19,9 @@
  *     wt_status code otherwise; it never throws or aborts.  The message of
  *     the last failure on the calling thread is available from
  *     wt_last_error().
- *   - Reentrant: no global mutable state except the per-thread error string
- *     and the per-device kernel-attribute cache (mutex protected).
+ *   - Reentrant: no global mutable state except the per-thread error string,
+ *     the per-device kernel-attribute cache and wt_autotune's result cache
+ *     (both mutex protected).
  */
 #ifndef WAVETORCH_HIP_H
 #define WAVETORCH_HIP_H
@@ -200,6 +201,27 @@ int wt_loop_adj(void *plan, const float *b, const float *c, const float *rho,
                 const float *x, const float *ybar, const float *fbar,
                 void *saved, void *temp, float *grad_c, float *grad_b,
                 float *grad_rho, float *grad_x, void *stream);
+
+/* Measured choice of the tuning for one problem shape (no counterpart in the
+ * reference, whose time loop is a Python loop, rnn.py:50-67): times
+ * wt_loop_fwd (and with with_grad the checkpointed wt_loop_adj) on the
+ * caller's b, c, rho, x (device buffers as for wt_loop_fwd) for candidate
+ * tilings, starting from the library's heuristic plan and moving one field
+ * at a time -- tiles per sample (strips x tcols) near the heuristic's,
+ * time-block depth, rows per thread, forward and adjoint separately -- then
+ * the coupled (depth, rows) pairs; keeps a change when it is > 0.5% faster;
+ * two passes or until budget_ms of wall time.  Each candidate's time is the
+ * best of `reps` timed runs (HIP events on `stream`) after one warm-up.
+ * `start` (may be NULL) supplies the fixed checkpoint interval (ck_every;
+ * 0 = 64).  Writes the winner (as wt_plan_resolve resolves it) to *out and
+ * its time to *best_ms (may be NULL).  The search allocates and frees its
+ * own device buffers (not for use under stream capture) and synchronises
+ * `stream`; results are cached per shape for the life of the process
+ * (thread-safe).  Python callers: wavetorch_amd.autotune.tune_native. */
+int wt_autotune(const wt_config *cfg, int32_t B, int32_t T, float cmax,
+                const wt_tuning *start, int32_t with_grad, const float *b,
+                const float *c, const float *rho, const float *x, int32_t reps,
+                double budget_ms, wt_tuning *out, double *best_ms, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Batched instances                                                         */

@@ -19,9 +19,13 @@
   -> cross-entropy -> adjoint -> rho at 150x150, batch 4, 3000 steps, for
   beta = 100 (every configuration's value) and beta = 1000 (the lens study
   steps beta up to 2000): forward probe series within 1e-5 of the
-  reference's fp32 op sequence (oracle/ref_torch.py), dJ/drho within 1e-4 of
-  the reference's fp32 autograd gradient and of the fp64 oracle chain, and
-  within 1e-4 of fp64 central differences on 8 pixels.
+  reference's fp32 op sequence (oracle/ref_torch.py) at beta 100; at beta
+  1000, where the reference's own fp32 forward is ~7e-5 from fp64 (the
+  projection amplifies one-ulp rounding differences of the blurred density),
+  at least as close to fp64 as the reference's and within that distance of
+  it; dJ/drho within 1e-4 of the reference's fp32 autograd gradient and of
+  the fp64 oracle chain, and within 1e-4 of fp64 central differences on 8
+  pixels, at both.
 
 The CPU legs (tests/parity_cpu.py: numpy and the C oracle) run in spawned
 worker processes, which never load the GPU stack.
@@ -223,6 +227,16 @@ def test_freeform_chain_train_150_b4(beta, cuda, pool):
             "grad_vs_fd": rel(gpx, fd), "fp64_vs_fd": rel(opx, fd)}
     _log(f"rho chain beta={beta}:", {k: f"{v:.2e}" for k, v in errs.items()})
     assert errs["fp64_vs_fd"] < 1e-5, errs  # the checker itself
-    assert errs["fwd_vs_ref32"] < 1e-5, errs
+    if beta <= 100.0:
+        assert errs["fwd_vs_ref32"] < 1e-5, errs
+    else:
+        # the projection's slope (beta / 2 at eta) turns one-ulp differences
+        # of the blurred density (torch-CPU and GPU blur/tanh round
+        # differently) into ~beta * 6e-8 relative changes of c: at beta 1000
+        # the reference's own fp32 forward is ~7e-5 from fp64.  The GPU's
+        # forward must be at least as close to fp64 as the reference's, and
+        # within that distance of the reference.
+        assert errs["fwd_vs_fp64"] <= 1.05 * errs["ref32_fwd_vs_fp64"], errs
+        assert errs["fwd_vs_ref32"] <= errs["ref32_fwd_vs_fp64"], errs
     assert errs["grad_vs_ref32"] < 1e-4 and errs["grad_vs_fp64"] < 1e-4, errs
     assert errs["grad_vs_fd"] < 1e-4, errs

@@ -59,7 +59,8 @@ def test_entry_points_reject_null_arguments():
              "wt_loop_adj", "wt_geom_freeform_c", "wt_geom_freeform_c_bwd",
              "wt_geom_holey_c", "wt_geom_holey_c_bwd", "wt_geom_holey_workspace_bytes",
              "wt_loss_power_ce", "wt_loss_power_ce_workspace_bytes", "wt_batch_plan_create", "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
-             "wt_batch_fwd", "wt_batch_adj", "wt_batch_fwd_adj", "wt_graph_launch"]
+             "wt_batch_fwd", "wt_batch_adj", "wt_batch_fwd_adj", "wt_graph_launch",
+             "wt_autotune"]
     for n in names:
         argtypes = _lib._SIGS[n][1]
         args = []

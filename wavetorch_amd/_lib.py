@@ -95,6 +95,10 @@ _SIGS = {
     "wt_loss_power_ce": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p]),
     "wt_loss_power_ce_workspace_bytes": (ctypes.c_int, [c_i32, c_i32, c_i32,
                                                         ctypes.POINTER(c_i64)]),
+    "wt_autotune": (ctypes.c_int, [ctypes.POINTER(WtConfig), c_i32, c_i32, c_f32,
+                                   ctypes.POINTER(WtTuning), c_i32, c_p, c_p, c_p, c_p, c_i32,
+                                   ctypes.c_double, ctypes.POINTER(WtTuning),
+                                   ctypes.POINTER(ctypes.c_double), c_p]),
     "wt_batch_plan_create": (ctypes.c_int, [c_i32, ctypes.POINTER(WtConfig),
                                             ctypes.POINTER(c_i32), ctypes.POINTER(c_i32),
                                             ctypes.POINTER(c_f32), ctypes.POINTER(WtTuning),

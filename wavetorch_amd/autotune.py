@@ -246,6 +246,49 @@ def tune(model, x, labels=None, steps=2, verbose=False, budget_s=300.0, rounds=2
     return best
 
 
+def tune_native(model, x, with_grad=True, reps=2, budget_s=120.0, store=True):
+    """The library's own search (``wt_autotune``, the C ABI's tuner: the same
+    fields, options and acceptance rule as :func:`tune`, timed with HIP events
+    on the loop kernels alone -- forward, and with ``with_grad`` the
+    checkpointed adjoint -- rather than on a whole training step) for
+    ``model`` on batch ``x`` at the model's checkpoint interval.  Returns the
+    winning tuning dict (also set as ``model.tuning``) and its time in ms;
+    with ``store`` the winner is cached under key(Nx, Ny, B, T, K) like
+    :func:`tune`'s."""
+    import ctypes
+    from . import _lib
+    B, T = x.shape
+    cfg = model.loop_config()
+    geom = model.cell.geom
+    with torch.no_grad():
+        c = geom.c.detach().contiguous()
+        b = geom.b.detach().contiguous()
+        rho = geom.rho.detach().contiguous() if cfg.nonlinear else None
+    xx = x.detach().contiguous()
+    K = int(model.checkpoint_every)
+    start = _lib.WtTuning(ck_every=K)
+    out = _lib.WtTuning()
+    ms = ctypes.c_double(0.0)
+    cc, keep = cfg.to_c()
+    with torch.cuda.device(x.device):
+        _lib.check(_lib.lib().wt_autotune(
+            ctypes.byref(cc), B, T, float(geom.cmax), ctypes.byref(start), int(bool(with_grad)),
+            b.data_ptr(), c.data_ptr(), rho.data_ptr() if rho is not None else None,
+            xx.data_ptr(), int(reps), float(budget_s * 1e3), ctypes.byref(out),
+            ctypes.byref(ms), _lib.stream_ptr(x.device)), "wt_autotune")
+    del keep
+    best = {k: v for k, v in out.as_dict().items() if v}
+    best.pop("ck_every", None)  # the model's K stays the model's
+    model.tuning = dict(best)
+    if store:
+        with _lock:
+            d = _load()
+            d[key(cfg.Nx, cfg.Ny, B, T, K, forward_only=not with_grad)] = {
+                "tuning": best, "ms": ms.value, "search": "wt_autotune"}
+            _store(d)
+    return best, ms.value
+
+
 # ------------------------------------------------------------ batched instances
 
 def _aligned_k(tb, atb, kmin=64):
