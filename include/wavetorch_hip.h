@@ -21,7 +21,8 @@
  *     wt_last_error().
  *   - Reentrant: no global mutable state except the per-thread error string,
  *     the per-device kernel-attribute cache and wt_autotune's result cache
- *     (both mutex protected).
+ *     (both mutex protected), and the static device workspace of
+ *     wt_loss_power_ce_static (shared by that entry point's calls on a device).
  */
 #ifndef WAVETORCH_HIP_H
 #define WAVETORCH_HIP_H
@@ -343,6 +344,19 @@ int wt_loss_power_ce(const float *y, const int64_t *labels, int32_t B, int32_t T
 /* Bytes of `work` that wt_loss_power_ce needs for (B, T, P) (no counterpart
  * in the reference). */
 int wt_loss_power_ce_workspace_bytes(int32_t B, int32_t T, int32_t P, int64_t *bytes);
+
+/* wt_loss_power_ce without a workspace argument: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * stream capture, fixed-shape serving loops, harnesses that forbid
+ * hipMalloc).  It runs the same three kernels, with the partial sums in a
+ * static device array of the library (1 MiB = 131072 doubles; each device
+ * the library runs on holds its own copy until the process ends), and
+ * allocates nothing.  Shapes needing more than B*C*P + B <= 131072 doubles
+ * (C = the chunk count, at most 16; e.g. B <= 2048 at P = 3) give
+ * WT_ENOMEM.  Calls that overlap on one device share the array and race;
+ * calls on one stream are ordered. */
+int wt_loss_power_ce_static(const float *y, const int64_t *labels, int32_t B, int32_t T,
+                            int32_t P, float *loss, float *grad_I, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Kernel timing (no counterpart in the reference, which has no profiler)    */

@@ -95,6 +95,7 @@ _SIGS = {
     "wt_loss_power_ce": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p]),
     "wt_loss_power_ce_workspace_bytes": (ctypes.c_int, [c_i32, c_i32, c_i32,
                                                         ctypes.POINTER(c_i64)]),
+    "wt_loss_power_ce_static": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p]),
     "wt_autotune": (ctypes.c_int, [ctypes.POINTER(WtConfig), c_i32, c_i32, c_f32,
                                    ctypes.POINTER(WtTuning), c_i32, c_p, c_p, c_p, c_p, c_i32,
                                    ctypes.c_double, ctypes.POINTER(WtTuning),

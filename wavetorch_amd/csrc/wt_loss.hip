@@ -25,6 +25,7 @@
 //  3. the batch mean.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "wt_internal.h"
 
@@ -139,21 +140,14 @@ extern "C" int wt_loss_power_ce_workspace_bytes(int32_t B, int32_t T, int32_t P,
   return WT_OK;
 }
 
-extern "C" int wt_loss_power_ce(const float *y, const int64_t *labels, int32_t B, int32_t T,
-                                int32_t P, float *loss, float *grad_I, void *work,
-                                void *stream) {
-  if (!y || !labels || !loss || !grad_I || !work) {
-    set_error("wt_loss_power_ce: invalid argument (null pointer)");
-    return WT_EINVAL;
-  }
-  if (!loss_shape_ok("wt_loss_power_ce", B, T, P)) return WT_EINVAL;
-  if (P > LOSS_MAX_P) {
-    set_error("wt_loss_power_ce: %d probes (at most %d)", P, LOSS_MAX_P);
-    return WT_EUNSUPPORTED;
-  }
-  hipStream_t s = (hipStream_t)stream;
+// The three launches on `work` (B*C*P + B doubles: partials, then per-sample
+// losses).  Every partial and per-sample loss is written before it is read,
+// so `work` needs no clearing.
+static int power_ce_launch(const float *y, const int64_t *labels, int32_t B, int32_t T,
+                           int32_t P, float *loss, float *grad_I, double *work,
+                           hipStream_t s) {
   const int C = loss_chunks(B, T);
-  double *part = (double *)work;
+  double *part = work;
   double *loss_b = part + (int64_t)B * C * P;
   hipLaunchKernelGGL(k_power_ce_partial, dim3(C, B), dim3(LOSS_NT), 0, s, y, (int64_t)T, P,
                      part);
@@ -164,4 +158,97 @@ extern "C" int wt_loss_power_ce(const float *y, const int64_t *labels, int32_t B
   hipLaunchKernelGGL(k_power_ce_mean, dim3(1), dim3(LOSS_NT), 0, s, loss_b, B, loss);
   WT_HIP(hipGetLastError());
   return WT_OK;
+}
+
+// Argument checks shared by both entry points.
+static int power_ce_check(const char *what, const float *y, const int64_t *labels, int32_t B,
+                          int32_t T, int32_t P, const float *loss, const float *grad_I) {
+  if (!y || !labels || !loss || !grad_I) {
+    set_error("%s: invalid argument (null pointer)", what);
+    return WT_EINVAL;
+  }
+  if (!loss_shape_ok(what, B, T, P)) return WT_EINVAL;
+  if (P > LOSS_MAX_P) {
+    set_error("%s: %d probes (at most %d)", what, P, LOSS_MAX_P);
+    return WT_EUNSUPPORTED;
+  }
+  return WT_OK;
+}
+
+extern "C" int wt_loss_power_ce(const float *y, const int64_t *labels, int32_t B, int32_t T,
+                                int32_t P, float *loss, float *grad_I, void *work,
+                                void *stream) {
+  if (!work) {
+    set_error("wt_loss_power_ce: invalid argument (null pointer)");
+    return WT_EINVAL;
+  }
+  const int st = power_ce_check("wt_loss_power_ce", y, labels, B, T, P, loss, grad_I);
+  if (st != WT_OK) return st;
+  return power_ce_launch(y, labels, B, T, P, loss, grad_I, (double *)work, (hipStream_t)stream);
+}
+
+// ------------------------------------------------ the allocation-free variant
+//
+// wt_loss_power_ce_static runs the same three kernels on a fixed device array
+// of this file instead of a caller-provided workspace, for callers that cannot
+// allocate device memory (code under stream capture, fixed-shape serving
+// loops).  Each device the library runs on holds its own copy of the array
+// until the process ends.  Its device address is looked up once per device
+// (hipGetSymbolAddress with that device current) and kept here; host code
+// never passes the array's own name to a kernel, since in host code that name
+// is a host-side stand-in, not a device address.  Calls that overlap on one
+// device share the array and race; calls on one stream are ordered.
+constexpr int64_t LOSS_STATIC_DOUBLES = 131072;  // 1 MiB
+static __device__ __align__(256) double scratch[LOSS_STATIC_DOUBLES];
+
+constexpr int LOSS_MAX_DEVICES = 64;
+static std::atomic<double *> scratch_ptr[LOSS_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+// Device address of this device's copy of `scratch`, looked up on the first
+// call on the device (the current device).
+static int scratch_of_current_device(const char *what, double **out) {
+  int dev = 0;
+  WT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= LOSS_MAX_DEVICES) {
+    set_error("%s: device %d (at most %d devices)", what, dev, LOSS_MAX_DEVICES);
+    return WT_EUNSUPPORTED;
+  }
+  double *p = scratch_ptr[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    p = scratch_ptr[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      void *addr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !addr) {
+        set_error("%s: hipGetSymbolAddress of the static workspace failed on device %d: %s",
+                  what, dev, hipGetErrorString(e));
+        return WT_EHIP;
+      }
+      p = (double *)addr;
+      scratch_ptr[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return WT_OK;
+}
+
+extern "C" int wt_loss_power_ce_static(const float *y, const int64_t *labels, int32_t B,
+                                       int32_t T, int32_t P, float *loss, float *grad_I,
+                                       void *stream) {
+  const char *what = "wt_loss_power_ce_static";
+  int st = power_ce_check(what, y, labels, B, T, P, loss, grad_I);
+  if (st != WT_OK) return st;
+  const int64_t need = (int64_t)B * loss_chunks(B, T) * P + B;
+  if (need > LOSS_STATIC_DOUBLES) {
+    set_error("%s: (B %d, T %d, P %d) needs %lld doubles of workspace, the static workspace "
+              "holds %lld (B * chunks * P + B; use wt_loss_power_ce with a workspace)",
+              what, B, T, P, (long long)need, (long long)LOSS_STATIC_DOUBLES);
+    return WT_ENOMEM;
+  }
+  double *work = nullptr;
+  st = scratch_of_current_device(what, &work);
+  if (st != WT_OK) return st;
+  return power_ce_launch(y, labels, B, T, P, loss, grad_I, work, (hipStream_t)stream);
 }
