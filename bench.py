@@ -29,6 +29,10 @@ Prints ONE JSON line:
   cpu_baseline   the torch-CPU restatement of the reference (oracle/ref_torch)
                  on a bounded sample of the same workload on this host's cores
   plain          the same step with the one-launch-per-step kernels
+
+`--dtype float64` times a training workload in double precision instead (the
+per-step float64 kernels; its roofline is algorithmic bytes over the launch
+time of the longer sweep, from HIP events).
 """
 from __future__ import annotations
 
@@ -344,6 +348,108 @@ def summarize_pmc(fetch_dir, write_dir, calib_bytes, extra_dirs=()):
     print("\n".join(lines))
 
 
+def run_f64(args, dev):
+    """The training step of a "train" workload in float64 (set_dtype('float64'),
+    the precision the reference's example configurations train in): geometry
+    and loss as torch ops in float64, the double-precision time loop
+    (k64_fwd one launch per step, the adjoint as k64_adj_a + k64_adj_b per
+    step over the stored fields).  HIP events on the launch stream bracket
+    the forward sweep and the backward of each timed step; the roofline is
+    that of the sweep that takes longer, at its algorithmic bytes (24 B per
+    cell-step forward: read u_n, u_{n-1}, write u_{n+1}; 32 B adjoint: read
+    lambda_{m+1}, lambda_{m+2}, u_m, write lambda_m)."""
+    from wavetorch_amd.utils import normalize_power, set_dtype
+    wl = WORKLOADS[args.workload]
+    set_dtype("float64")
+    try:
+        model, x, labels = build(dev, None, args.workload)
+        x = x.double()
+        B, T = x.shape
+        Nx, Ny = model.cell.geom.domain_shape
+        cells = B * Nx * Ny * T
+        opt = torch.optim.Adam(model.parameters(), lr=1e-3)
+        for _ in range(args.warmup):
+            train_step(model, x, labels, opt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            loss = train_step(model, x, labels, opt)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        log(f"[bench] float64: {dt * 1e3:.2f} ms/step  loss {float(loss.detach()):.4f}")
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        fwd_ms = bwd_ms = 0.0
+        for _ in range(args.steps):
+            opt.zero_grad(set_to_none=True)
+            ev[0].record()
+            y = model(x)
+            ev[1].record()
+            torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels).backward()
+            ev[2].record()
+            opt.step()
+            torch.cuda.synchronize()
+            fwd_ms += ev[0].elapsed_time(ev[1]) / args.steps
+            bwd_ms += ev[1].elapsed_time(ev[2]) / args.steps
+        log(f"[bench] float64 sweeps: forward {fwd_ms:.2f} ms, backward {bwd_ms:.2f} ms")
+        fwd = fwd_ms >= bwd_ms
+        kname = "k64_fwd" if fwd else "k64_adj_a + k64_adj_b (one adjoint step)"
+        avg_s = (fwd_ms if fwd else bwd_ms) / 1e3 / T
+        per_launch = B * Nx * Ny
+        byts = (24.0 if fwd else 32.0) * per_launch
+        eff = {"basis": f"{byts / per_launch:g} algorithmic B per cell-step x cells / launch "
+                        "time", "achieved": round(byts / avg_s / 1e9, 1), "unit": "GB/s",
+               "frac_of_hbm_peak": round(byts / avg_s / PEAK_HBM, 4)}
+        # counter bytes (profiles/traffic_<workload>_f64.json, --summarize-pmc)
+        knames = ["wt::k64_fwd"] if fwd else ["wt::k64_adj_a", "wt::k64_adj_b"]
+        ks = [kernel_counters(f"{args.workload}_f64", k, "float64") for k in knames]
+        traffic_b = (sum(k["hbm_bytes_per_launch"] for k in ks)
+                     if all(k.get("hbm_bytes_per_launch") for k in ks) else None)
+        if traffic_b:
+            roof = {"achieved": round(traffic_b / avg_s / 1e9, 1),
+                    "frac": round(traffic_b / avg_s / PEAK_HBM, 4),
+                    "basis": "HBM bytes counted by FETCH_SIZE + WRITE_SIZE (calibrated) per "
+                             "launch / launch time from HIP events over the sweep"}
+        else:
+            roof = {"achieved": eff["achieved"], "frac": eff["frac_of_hbm_peak"],
+                    "basis": eff["basis"]}
+        out = {
+            "metric": wl["metric"].replace("Gcell-steps/s (", "Gcell-steps/s (float64, "),
+            "value": round(cells / dt / 1e9, 3),
+            "unit": "Gcell-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(dt * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic (seeded inputs and rho, see wavetorch_amd/workloads.py)",
+            "config": {"workload": f"{args.workload} in float64: {wl['desc']}", "B": B,
+                       "Nx": Nx, "Ny": Ny, "T": T},
+            "roofline": {"kernel": kname, "bound": "hbm", "achieved": roof["achieved"],
+                         "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": roof["frac"],
+                         "basis": roof["basis"], "avg_launch_us": round(avg_s * 1e6, 2),
+                         "traffic": traffic_b, "effective_bw": eff,
+                         "forward_ms_per_step": round(fwd_ms, 3),
+                         "backward_ms_per_step": round(bwd_ms, 3)},
+            "loss_finite": bool(np.isfinite(float(loss.detach()))),
+        }
+        if args.meta_out:
+            with open(args.meta_out, "w") as f:
+                json.dump({"tuning": "float64", "workload": f"{args.workload}_f64"}, f)
+        if args.calib_copy > 0:
+            a = torch.ones(args.calib_copy * 2 ** 20 // 4, device=dev, dtype=torch.float32)
+            bb = torch.empty_like(a)
+            for _ in range(3):
+                bb.copy_(a)
+            torch.cuda.synchronize()
+            del a, bb
+        del model, x, labels, opt, y
+        if not args.no_cpu:
+            log("[bench] cpu baseline (float64) ...")
+            out["cpu_baseline"] = cpu_baseline(args.workload)
+    finally:
+        set_dtype("float32")
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(workload="train_150_b64", budget_s=15.0):
     """The reference's op sequence (oracle/ref_torch: the torch-CPU restatement
     of rnn.py/cell.py/source.py/probe.py/geom.py with its autograd tape) on
@@ -352,6 +458,7 @@ def cpu_baseline(workload="train_150_b64", budget_s=15.0):
     from wavetorch_amd import workloads
     spec = WORKLOADS[workload]
     model, x, labels = getattr(workloads, workload)()
+    x = x.to(torch.get_default_dtype())  # float64 under set_dtype('float64')
     g, cell = model.cell.geom, model.cell
     torch.set_num_threads(os.cpu_count() or 1)
     train = spec["kind"] == "train"
@@ -389,9 +496,11 @@ def cpu_baseline(workload="train_150_b64", budget_s=15.0):
     cells = B * Nx * Ny * T
     what = "fwd+autograd bwd through the restated TimeStep.backward" if train else \
         "forward keeping every field (rnn.py:66-70)"
+    prec = "float64, " if x.dtype == torch.float64 else ""
     return {"value": cells / el / 1e9, "unit": "Gcell-steps/s", "cores": torch.get_num_threads(),
             "kind": "port",
-            "sample": f"batch {B}, {Nx}x{Ny}, first {T} of {Tfull} steps, {what}, {el:.1f} s"}
+            "sample": f"{prec}batch {B}, {Nx}x{Ny}, first {T} of {Tfull} steps, {what}, "
+                      f"{el:.1f} s"}
 
 
 def _holey_torch_ops(hx, hy, hr, shape, eta, beta, c0, c1):
@@ -619,6 +728,8 @@ def main():
     ap.add_argument("--workload", default="train_150_b64", choices=sorted(WORKLOADS),
                     help="configuration to time (default: the headline training step)")
     ap.add_argument("--tuning", type=str, default=None, help="JSON wt_tuning override")
+    ap.add_argument("--dtype", choices=("float32", "float64"), default="float32",
+                    help="float64: the training step in double precision (train workloads)")
     ap.add_argument("--loss", choices=("torch", "fused"), default="torch",
                     help="loss head of the training step: the reference's torch ops (default) "
                          "or the fused wt_loss_power_ce kernel")
@@ -648,6 +759,11 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.init()
     wl = WORKLOADS[args.workload]
+    if args.dtype == "float64":
+        if wl["kind"] != "train":
+            ap.error("--dtype float64 times the training workloads")
+        run_f64(args, dev)
+        return
     if wl["kind"] == "batch":
         run_batch(args, dev)
         return

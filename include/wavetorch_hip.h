@@ -225,6 +225,45 @@ int wt_autotune(const wt_config *cfg, int32_t B, int32_t T, float cmax,
                 double budget_ms, wt_tuning *out, double *best_ms, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* The time loop in double precision (set_dtype('float64'), utils.py:14-20)  */
+/* ------------------------------------------------------------------------ */
+
+/* Scalars of the float64 loop, which wt_config holds only as float: dt
+ * (cell.py:60), h (geom.py:20) and the nonlinear coefficients (cell.py:60-64;
+ * c_min / c_max of the clamp extension). */
+typedef struct wt_scalars_f64 {
+  double dt, h, sat_b0, sat_uth, c_nl, c_min, c_max;
+} wt_scalars_f64;
+
+/* Bytes of the workspaces of wt_loop_fwd_f64 / wt_loop_adj_f64 for a plan
+ * made by wt_plan_create for the same configuration, batch and length:
+ *   saved (with_grad): every field u_n, n = -1..T, ((T+2)*B*Nx*Ny doubles,
+ *          what the reference's autograd tape keeps; no checkpoint recompute)
+ *   temp:  three fields (forward without gradients), or the adjoint's
+ *          lambdas, coefficient products and gradient accumulators. */
+int wt_workspace_bytes_f64(const void *plan, int32_t with_grad, int64_t *saved,
+                           int64_t *temp);
+
+/* wt_loop_fwd in float64 (rnn.py:21-72 under set_dtype('float64')): one
+ * launch per step, the reference's operation order in double precision.
+ * Arguments as wt_loop_fwd, all arrays double; `sc` gives dt, h and the
+ * nonlinear coefficients in double. */
+int wt_loop_fwd_f64(void *plan, const wt_scalars_f64 *sc, const double *b, const double *c,
+                    const double *rho, const double *x, double *y, double *fields,
+                    int32_t field_stride, int32_t with_grad, void *saved, void *temp,
+                    void *stream);
+
+/* wt_loop_adj in float64: the reverse sweep over the stored fields (the
+ * autograd replay of cell.py:26-44 over rnn.py:50-70 in double precision).
+ * Arguments as wt_loop_adj (no x: the fields are stored), all arrays double;
+ * output buffers are overwritten; grad_b is dJ/db for the linear cell as well;
+ * fbar, when given, holds every step (field_stride 1). */
+int wt_loop_adj_f64(void *plan, const wt_scalars_f64 *sc, const double *b, const double *c,
+                    const double *rho, const double *ybar, const double *fbar, void *saved,
+                    void *temp, double *grad_c, double *grad_b, double *grad_rho,
+                    double *grad_x, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Batched instances                                                         */
 /* ------------------------------------------------------------------------ */
 /* Many independent linear WaveRNN problems of uneven grid size and length

@@ -58,9 +58,11 @@ def test_entry_points_reject_null_arguments():
              "wt_plan_create", "wt_plan_tuning", "wt_workspace_bytes", "wt_loop_fwd",
              "wt_loop_adj", "wt_geom_freeform_c", "wt_geom_freeform_c_bwd",
              "wt_geom_holey_c", "wt_geom_holey_c_bwd", "wt_geom_holey_workspace_bytes",
-             "wt_loss_power_ce", "wt_loss_power_ce_workspace_bytes", "wt_batch_plan_create", "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
+             "wt_loss_power_ce", "wt_loss_power_ce_workspace_bytes", "wt_batch_plan_create",
+             "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
              "wt_batch_fwd", "wt_batch_adj", "wt_batch_fwd_adj", "wt_graph_launch",
-             "wt_autotune"]
+             "wt_autotune", "wt_loss_power_ce_static", "wt_workspace_bytes_f64",
+             "wt_loop_fwd_f64", "wt_loop_adj_f64"]
     for n in names:
         argtypes = _lib._SIGS[n][1]
         args = []

@@ -50,6 +50,12 @@ class WtTuning(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class WtScalarsF64(ctypes.Structure):
+    """Mirror of ``wt_scalars_f64`` (the float64 loop's scalars)."""
+    _fields_ = [(n, ctypes.c_double) for n in ("dt", "h", "sat_b0", "sat_uth", "c_nl", "c_min",
+                                               "c_max")]
+
+
 class WtBatchIO(ctypes.Structure):
     """Mirror of ``wt_batch_group_io``."""
     _fields_ = [("b", c_p), ("c", c_p), ("x", c_p), ("y", c_p), ("ybar", c_p),
@@ -96,6 +102,12 @@ _SIGS = {
     "wt_loss_power_ce_workspace_bytes": (ctypes.c_int, [c_i32, c_i32, c_i32,
                                                         ctypes.POINTER(c_i64)]),
     "wt_loss_power_ce_static": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p]),
+    "wt_workspace_bytes_f64": (ctypes.c_int, [c_p, c_i32, ctypes.POINTER(c_i64),
+                                              ctypes.POINTER(c_i64)]),
+    "wt_loop_fwd_f64": (ctypes.c_int, [c_p, ctypes.POINTER(WtScalarsF64), c_p, c_p, c_p, c_p, c_p,
+                                       c_p, c_i32, c_i32, c_p, c_p, c_p]),
+    "wt_loop_adj_f64": (ctypes.c_int, [c_p, ctypes.POINTER(WtScalarsF64), c_p, c_p, c_p, c_p, c_p,
+                                       c_p, c_p, c_p, c_p, c_p, c_p, c_p]),
     "wt_autotune": (ctypes.c_int, [ctypes.POINTER(WtConfig), c_i32, c_i32, c_f32,
                                    ctypes.POINTER(WtTuning), c_i32, c_p, c_p, c_p, c_p, c_i32,
                                    ctypes.c_double, ctypes.POINTER(WtTuning),

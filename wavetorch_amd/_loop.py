@@ -51,6 +51,14 @@ class LoopConfig:
         return cfg, (src, taps)  # keep the arrays alive while cfg is used
 
 
+def scalars_f64(cfg: LoopConfig):
+    """``wt_scalars_f64`` of a configuration: dt, h and the nonlinear
+    coefficients in double precision (``wt_config`` holds them as float)."""
+    cmin, cmax = cfg.c_clamp if cfg.c_clamp is not None else (0.0, 0.0)
+    return _lib.WtScalarsF64(dt=cfg.dt, h=cfg.h, sat_b0=cfg.sat_b0, sat_uth=cfg.sat_uth,
+                             c_nl=cfg.c_nl, c_min=cmin, c_max=cmax)
+
+
 def resolve(cfg: LoopConfig, B: int, T: int, cmax: float, tuning=None):
     """Host-only plan resolution (``wt_plan_resolve``): validates ``cfg``
     (ValueError on CFL or bad indices, as the reference's constructors raise)
@@ -96,6 +104,14 @@ class Plan:
                    "wt_workspace_bytes")
         self.temp_bytes_nograd = t0.value
 
+    def bytes_f64(self, with_grad: bool):
+        """(saved, temp) bytes of the float64 loop (wt_workspace_bytes_f64)."""
+        s0, t0 = _lib.c_i64(), _lib.c_i64()
+        _lib.check(_lib.lib().wt_workspace_bytes_f64(self.handle, int(with_grad),
+                                                     ctypes.byref(s0), ctypes.byref(t0)),
+                   "wt_workspace_bytes_f64")
+        return s0.value, t0.value
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
@@ -112,12 +128,17 @@ def _buf(nbytes, device):
 
 def _check_finite(t, flag, slot, stream):
     if t is not None and flag is not None:
+        if t.dtype == torch.float64:  # on the device as well, no host synchronisation
+            flag[slot] = torch.maximum(flag[slot], (~torch.isfinite(t)).any().to(flag.dtype))
+            return
         _lib.check(_lib.lib().wt_check_finite(t.data_ptr(), t.numel(), flag.data_ptr() + 4 * slot,
                                               stream), "wt_check_finite")
 
 
 class WaveLoop(torch.autograd.Function):
     """y = WaveRNN time loop (x, c, b, rho); backward = checkpointed adjoint.
+    float64 inputs take the double-precision loop (wt_loop_fwd_f64 /
+    wt_loop_adj_f64: every field kept, no recompute).
 
     ``nonfinite``: None, or a device int32 tensor of two entries: entry 0 is
     set to 1 when the forward's output holds a NaN or infinity, entry 1 when
@@ -137,19 +158,34 @@ class WaveLoop(torch.autograd.Function):
         need = ctx.needs_input_grad
         with_grad = bool(need[0] or need[1] or need[2] or (rho is not None and need[3]))
         ntap = len(plan.cfg.taps)
+        f64 = c.dtype == torch.float64
+        dtype = torch.float64 if f64 else torch.float32
+        for t_, name in ((x, "x"), (c, "c"), (b, "b"), (rho_c, "rho")):
+            if t_ is not None and t_.dtype != dtype:
+                raise TypeError(f"wavetorch_amd: {name} is {t_.dtype}, the loop runs in {dtype}")
         y = None
         fields = None
         if output_fields:
-            fields = torch.empty((B, T // field_stride, Nx, Ny), dtype=torch.float32, device=dev)
+            fields = torch.empty((B, T // field_stride, Nx, Ny), dtype=dtype, device=dev)
         else:
-            y = torch.empty((B, T, ntap), dtype=torch.float32, device=dev)
-        saved = _buf(plan.saved_bytes, dev) if with_grad else None
-        temp = _buf(plan.temp_bytes_nograd, dev)  # the forward's scratch, with or without grad
+            y = torch.empty((B, T, ntap), dtype=dtype, device=dev)
         s = _lib.stream_ptr(dev)
-        _lib.check(_lib.lib().wt_loop_fwd(
-            plan.handle, b.data_ptr(), c.data_ptr(), _lib.ptr(rho_c), x.data_ptr(), _lib.ptr(y),
-            _lib.ptr(fields), int(field_stride), int(with_grad), _lib.ptr(saved),
-            temp.data_ptr(), s), "wt_loop_fwd")
+        if f64:
+            sb, tb = plan.bytes_f64(with_grad)
+            saved = _buf(sb, dev) if with_grad else None
+            temp = _buf(plan.bytes_f64(False)[1], dev)
+            sc = scalars_f64(plan.cfg)
+            _lib.check(_lib.lib().wt_loop_fwd_f64(
+                plan.handle, ctypes.byref(sc), b.data_ptr(), c.data_ptr(), _lib.ptr(rho_c),
+                x.data_ptr(), _lib.ptr(y), _lib.ptr(fields), int(field_stride), int(with_grad),
+                _lib.ptr(saved), temp.data_ptr(), s), "wt_loop_fwd_f64")
+        else:
+            saved = _buf(plan.saved_bytes, dev) if with_grad else None
+            temp = _buf(plan.temp_bytes_nograd, dev)  # the forward's scratch, with or without grad
+            _lib.check(_lib.lib().wt_loop_fwd(
+                plan.handle, b.data_ptr(), c.data_ptr(), _lib.ptr(rho_c), x.data_ptr(),
+                _lib.ptr(y), _lib.ptr(fields), int(field_stride), int(with_grad),
+                _lib.ptr(saved), temp.data_ptr(), s), "wt_loop_fwd")
         del temp
         _check_finite(fields if output_fields else y, nonfinite, 0, s)
         ctx.nonfinite = nonfinite
@@ -172,20 +208,30 @@ class WaveLoop(torch.autograd.Function):
         dev = x.device
         need = ctx.needs_input_grad
         Nx, Ny = plan.cfg.Nx, plan.cfg.Ny
-        g = gout.contiguous()
-        gc = torch.empty((Nx, Ny), dtype=torch.float32, device=dev) if need[1] else None
-        gb = torch.empty((Nx, Ny), dtype=torch.float32, device=dev) if need[2] else None
-        grho = torch.empty((Nx, Ny), dtype=torch.float32, device=dev) \
+        f64 = c.dtype == torch.float64
+        dtype = c.dtype
+        g = gout.to(dtype).contiguous()
+        gc = torch.empty((Nx, Ny), dtype=dtype, device=dev) if need[1] else None
+        gb = torch.empty((Nx, Ny), dtype=dtype, device=dev) if need[2] else None
+        grho = torch.empty((Nx, Ny), dtype=dtype, device=dev) \
             if (rho is not None and need[3]) else None
         gx = torch.empty_like(x) if need[0] else None
-        temp = _buf(plan.temp_bytes, dev)
         s = _lib.stream_ptr(dev)
-        _lib.check(_lib.lib().wt_loop_adj(
-            plan.handle, b.data_ptr(), c.data_ptr(), _lib.ptr(rho), x.data_ptr(),
-            None if ctx.output_fields else g.data_ptr(),
-            g.data_ptr() if ctx.output_fields else None,
-            ctx.ws.data_ptr(), temp.data_ptr(), _lib.ptr(gc), _lib.ptr(gb), _lib.ptr(grho),
-            _lib.ptr(gx), s), "wt_loop_adj")
+        ybar = None if ctx.output_fields else g.data_ptr()
+        fbar = g.data_ptr() if ctx.output_fields else None
+        if f64:
+            temp = _buf(plan.bytes_f64(True)[1], dev)
+            sc = scalars_f64(plan.cfg)
+            _lib.check(_lib.lib().wt_loop_adj_f64(
+                plan.handle, ctypes.byref(sc), b.data_ptr(), c.data_ptr(), _lib.ptr(rho), ybar,
+                fbar, ctx.ws.data_ptr(), temp.data_ptr(), _lib.ptr(gc), _lib.ptr(gb),
+                _lib.ptr(grho), _lib.ptr(gx), s), "wt_loop_adj_f64")
+        else:
+            temp = _buf(plan.temp_bytes, dev)
+            _lib.check(_lib.lib().wt_loop_adj(
+                plan.handle, b.data_ptr(), c.data_ptr(), _lib.ptr(rho), x.data_ptr(), ybar, fbar,
+                ctx.ws.data_ptr(), temp.data_ptr(), _lib.ptr(gc), _lib.ptr(gb), _lib.ptr(grho),
+                _lib.ptr(gx), s), "wt_loop_adj")
         ctx.ws = None
         for gt in (gx, gc, gb, grho):
             _check_finite(gt, ctx.nonfinite, 1, s)

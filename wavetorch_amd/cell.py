@@ -31,8 +31,35 @@ def _check_dev(*ts):
                                       f"(got {t.dtype})")
 
 
+def _time_step_float64(b, c, y1, y2, dt, h):
+    """The leapfrog step of cell.py:12-17 in float64 as torch ops on the
+    device (5-point Laplacian with zero padding as a conv2d, operators.py:5-11);
+    autograd differentiates it, which equals the reference's hand-written
+    TimeStep.backward (cell.py:26-44)."""
+    for t in (b, c, y1, y2):
+        if not t.is_cuda:
+            raise RuntimeError("wavetorch_amd: the time step runs on the HIP device only; "
+                               "move the tensors/model with .to('cuda')")
+    dt = torch.as_tensor(dt, dtype=torch.float64, device=y1.device)
+    h = torch.as_tensor(h, dtype=torch.float64, device=y1.device)
+    stencil = torch.tensor([[0.0, 1.0, 0.0], [1.0, -4.0, 1.0], [0.0, 1.0, 0.0]],
+                           dtype=torch.float64, device=y1.device)
+    kern = (h ** (-2) * stencil).unsqueeze(0).unsqueeze(0)
+    lap = torch.nn.functional.conv2d(y1.unsqueeze(1), kern, padding=1).squeeze(1)
+    inv = (dt ** -2 + b * dt ** -1).pow(-1)
+    return inv * (2 / dt ** 2 * y1 - (dt ** -2 - b * dt ** -1) * y2 + c.pow(2) * lap)
+
+
 class TimeStep(torch.autograd.Function):
-    """One leapfrog step y = _time_step(b, c, y1, y2, dt, h) (cell.py:12-44)."""
+    """One leapfrog step y = _time_step(b, c, y1, y2, dt, h) (cell.py:12-44).
+    float64 tensors take ``_time_step_float64`` (call ``TimeStep.apply`` as
+    usual)."""
+
+    @staticmethod
+    def apply(b, c, y1, y2, dt, h):  # noqa: D102 (the reference's call form)
+        if y1.dtype == torch.float64:
+            return _time_step_float64(b, c, y1, y2, dt, h)
+        return super(TimeStep, TimeStep).apply(b, c, y1, y2, dt, h)
 
     @staticmethod
     def forward(ctx, b, c, y1, y2, dt, h):

@@ -23,7 +23,7 @@ PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "_lib")
 LIBNAME = "libwavetorch_hip.so"
 SOURCES = ["wt_runtime.hip", "wt_plain.hip", "wt_tiled.hip", "wt_colseg.hip", "wt_geom.hip",
-           "wt_loss.hip", "wt_autotune.hip"]
+           "wt_loss.hip", "wt_autotune.hip", "wt_f64.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -136,7 +136,17 @@ struct NlAdjBufs {
   int64_t rec_slice;   // stride of one (sample, step) slice of the record
 };
 
-// Optional phase timestamps// Optional phase timestamps (diagnostic builds compiled with
+// Read-only view of a plan's configuration and device tables, for code
+// outside the runtime (the double-precision loop).
+struct PlanView {
+  int B, T, Nx, Ny;
+  int nlf;  // nonlinear flags as in FwdArgs (bit2 also for a clamp-only cell)
+  int nspx, ntap, nslot;
+  const int32_t *spx, *taps, *spmap, *mult;
+};
+int plan_view(const void *plan, PlanView *out);
+
+// Optional phase timestamps (diagnostic builds compiled with
 // -DWT_PHASE_TIMING): thread 0 of every workgroup records the 100 MHz wall
 // clock at kernel start, after set-up, after the step loop and at the end
 // into phase_ts[(blockIdx.x) * 8 + i].

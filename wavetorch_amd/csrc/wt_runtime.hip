@@ -869,6 +869,18 @@ static int plan_upload(Plan *p) {
   return WT_OK;
 }
 
+namespace wt {
+int plan_view(const void *plan, PlanView *out) {
+  const Plan *p = (const Plan *)plan;
+  if (!p || !out) return WT_EINVAL;
+  out->B = p->B; out->T = p->T; out->Nx = p->Nx; out->Ny = p->Ny;
+  out->nlf = p->nl | (p->clamp_only ? 4 : 0);
+  out->nspx = p->nspx; out->ntap = p->ntap; out->nslot = p->nslot;
+  out->spx = p->d_spx; out->taps = p->d_taps; out->spmap = p->d_spmap; out->mult = p->d_mult;
+  return WT_OK;
+}
+}  // namespace wt
+
 extern "C" {
 
 int wt_plan_destroy(void *plan) {

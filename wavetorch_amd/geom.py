@@ -128,10 +128,31 @@ class WaveGeometryHoley(WaveGeometry):
         return cached[1]
 
     def _holes(self, c0, c1):
+        if self.x.dtype == torch.float64:
+            _require_device(self.x, "WaveGeometryHoley")
+            return self._holes_float64(c0, c1)
         _require_hip(self.x, "WaveGeometryHoley")
         eta, beta, _, _ = self._host_params()
         Nx, Ny = self.domain_shape
         return _HoleyC.apply(self.x, self.y, self.r, Nx, Ny, eta, beta, c0, c1)
+
+    def _holes_float64(self, c0, c1):
+        """geom.py:109-135 in float64 on the device: the hole sum with the
+        reference's torch ops, projection, c0 + (c1 - c0) * rho (c0 = 0,
+        c1 = 1 gives rho itself)."""
+        eta, beta, _, _ = self._host_params()
+        Nx, Ny = self.domain_shape
+        dev, dt = self.x.device, self.x.dtype
+        xg, yg = torch.meshgrid(torch.arange(0, Nx, dtype=dt, device=dev),
+                                torch.arange(0, Ny, dtype=dt, device=dev), indexing="ij")
+        acc = torch.zeros((Nx, Ny), dtype=dt, device=dev)
+        for ri, xi, yi in zip(self.r, self.x, self.y):
+            d = torch.sqrt((xg - xi).pow(2) + (yg - yi).pow(2))
+            acc = acc + torch.exp(-d / ri)
+        rho = _projection(acc, eta, beta)
+        if c0 == 0.0 and c1 == 1.0:
+            return rho
+        return c0 + (c1 - c0) * rho
 
     def _rho(self):
         """Projected density P(sum_h exp(-d_h / r_h)) (geom.py:109-128), gfx950
@@ -240,6 +261,19 @@ class _FreeFormC(torch.autograd.Function):
         return g, None, None, None, None, None, None, None
 
 
+def _require_device(t, what):
+    if not t.is_cuda:
+        raise RuntimeError(
+            f"wavetorch_amd: {what} runs on the HIP device only; move the model with .to('cuda')")
+
+
+def _projection(x, eta, beta):
+    """The tanh projection of geom.py:217-222 with host-side tanh(beta*eta)
+    and tanh(beta*(1-eta)) (numpy, as the reference evaluates them)."""
+    te = np.tanh(beta * eta)
+    return (te + torch.tanh(beta * (x - eta))) / (te + np.tanh(beta * (1 - eta)))
+
+
 def _require_hip(t, what):
     if not t.is_cuda:
         raise RuntimeError(
@@ -343,8 +377,23 @@ class WaveGeometryFreeForm(WaveGeometry):
 
     @property
     def c(self):
-        """c0 + (c1-c0) * P(Blur^N(rho)) (geom.py:231-233), gfx950 kernels."""
+        """c0 + (c1-c0) * P(Blur^N(rho)) (geom.py:231-233), gfx950 kernels
+        (float32); in float64 the reference's torch ops on the device."""
+        if self.rho.dtype == torch.float64:
+            _require_device(self.rho, "WaveGeometryFreeForm.c")
+            return self._c_float64()
         _require_hip(self.rho, "WaveGeometryFreeForm.c")
         k = self.blur_kernel[0, 0].to(device=self.rho.device, dtype=torch.float32)
         blur_N, eta, beta, c0, c1 = self._host_params()
         return _FreeFormC.apply(self.rho, k, self._blur_radius, blur_N, eta, beta, c0, c1)
+
+    def _c_float64(self):
+        """geom.py:207-233 in float64 on the device: blur_N passes of conv2d
+        with the disk kernel (zero padding), projection, c0 + (c1 - c0) P."""
+        blur_N, eta, beta, c0, c1 = self._host_params()
+        rho = self.rho
+        k = self.blur_kernel.to(device=rho.device, dtype=rho.dtype)
+        for _ in range(blur_N):
+            rho = torch.nn.functional.conv2d(rho.unsqueeze(0).unsqueeze(0), k,
+                                             padding=k.shape[-1] // 2).squeeze()
+        return c0 + (c1 - c0) * _projection(rho, eta, beta)

@@ -144,8 +144,12 @@ class WaveRNN(torch.nn.Module):
         if not x.is_cuda:
             raise RuntimeError("wavetorch_amd.WaveRNN runs on the HIP device only; move the "
                                "model and inputs with .to('cuda')")
-        if x.dtype != torch.float32:
-            raise NotImplementedError(f"wavetorch_amd computes in float32 (got {x.dtype})")
+        if x.dtype not in (torch.float32, torch.float64):
+            raise NotImplementedError(f"wavetorch_amd computes in float32 or float64 "
+                                      f"(got {x.dtype})")
+        if x.dtype != self.cell.geom.b.dtype:
+            raise TypeError(f"wavetorch_amd: input is {x.dtype}, the model's buffers are "
+                            f"{self.cell.geom.b.dtype} (set_dtype before building the model)")
         B, T = x.shape[0], x.shape[1]
         fields_mode = len(self.probes) == 0 or output_fields
         stride = int(self.field_stride) if fields_mode else 1

@@ -17,8 +17,9 @@ def to_tensor(x, dtype=None):
 
 
 def set_dtype(dtype=None):
-    """utils.py:14-20.  The HIP kernels compute in float32; float64 is accepted
-    for module construction but the time loop raises for it."""
+    """utils.py:14-20.  Modules built after set_dtype('float64') hold float64
+    buffers and run the double-precision time loop (one launch per step,
+    every field kept); float32 runs the temporally blocked kernels."""
     if dtype == 'float32' or dtype is None:
         torch.set_default_dtype(torch.float32)
     elif dtype == 'float64':
