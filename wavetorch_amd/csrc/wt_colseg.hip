@@ -47,16 +47,34 @@ struct CsGeom {
   int base;               // LDS index of this thread's first cell
 };
 
-template <int R>
+// LAYOUT: take the thread-mapping width and LDS column stride the host chose
+// (cs_layout, descriptor words 12 and 13; the plain mapping unless they fit
+// this launch's rows per thread and threads) -- the forward kernels, whose
+// LDS allocation holds the padded stride.  Otherwise (the adjoints, which
+// wait little on LDS) the plain mapping t -> (t / RW, t % RW).
+template <int R, bool LAYOUT = false>
 __device__ __forceinline__ CsGeom cs_geom(const int32_t *td, int tid) {
   CsGeom g;
   g.R0 = td[0]; g.RH = td[1]; g.C0 = td[2]; g.RW = td[3];
   g.or0 = td[4]; g.or1 = td[5]; g.oc0 = td[6]; g.oc1 = td[7];
   const int nseg = (g.RH + R - 1) / R;
-  g.k = tid / g.RW;
-  g.c = tid - g.k * g.RW;
-  g.active = g.k < nseg;
-  g.CS = (nseg * R + 2) | 1;
+  if constexpr (LAYOUT) {
+    int RWm = td[12], CS = td[13];
+    if (RWm < g.RW || CS < nseg * R + 2 || CS > ((nseg * R + 2) | 1) + cs_lds_slack(R) ||
+        RWm * nseg > (int)blockDim.x) {
+      RWm = g.RW;
+      CS = (nseg * R + 2) | 1;
+    }
+    g.CS = CS;
+    g.k = tid / RWm;
+    g.c = tid - g.k * RWm;
+    g.active = g.k < nseg && g.c < g.RW;
+  } else {
+    g.k = tid / g.RW;
+    g.c = tid - g.k * g.RW;
+    g.active = g.k < nseg;
+    g.CS = (nseg * R + 2) | 1;
+  }
   g.base = (g.c + 1) * g.CS + g.k * R + 1;
   return g;
 }
@@ -207,7 +225,7 @@ __device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
   // sample's last step, so that the adjoint finds every record it reads
   if (a.Tlim && (REC ? a.rec_n0 : a.n0) >= a.Tlim[b]) return;
   const int32_t *td = a.tiles + ti * TILE_W;
-  const CsGeom g = cs_geom<R>(td, tid);
+  const CsGeom g = cs_geom<R, true>(td, tid);
   const int NBUF = (g.RW + 2) * g.CS;
   float *buf0 = smem, *buf1 = smem + NBUF;
   float *xs = smem + 2 * NBUF;  // the input samples of this launch
@@ -953,15 +971,25 @@ bool cs_variant_exists(int nt, int r) {
   return false;
 }
 
-// LDS bytes of a region with RH rows and RW columns for rows-per-thread R.
-size_t cs_shmem_bytes(int R, int RH, int RW) {
+// Largest LDS column stride of a region of at most RH rows; `padded`:
+// with cs_layout's slack (forward tiles, whose descriptors carry a layout),
+// so that every tile of the launch fits the allocation.
+static int cs_stride_alloc(int R, int RH, bool padded) {
   int nseg = (RH + R - 1) / R;
-  int CS = (nseg * R + 2) | 1;
+  return ((nseg * R + 2) | 1) + (padded ? cs_lds_slack(R) : 0);
+}
+
+// LDS bytes of a region with RH rows and RW columns for rows-per-thread R.
+size_t cs_shmem_bytes(int R, int RH, int RW, bool padded) {
+  int CS = cs_stride_alloc(R, RH, padded);
   // field double buffer + staged inputs (forward) or probe seeds (adjoint)
   return ((size_t)2 * (RW + 2) * CS + (size_t)CS_MAX_SEEDS * CS_MAX_TBLOCK) * sizeof(float);
 }
 
-// threads a region needs
+// Threads of a launch whose largest region is RH x RW: whole waves for the
+// plain mapping.  cs_layout pads columns only within these waves: an extra
+// wave costs a VALU-bound block more than the bank conflicts it removes
+// (measured: 12 -> 13 waves made the nonlinear forward 13% slower).
 int cs_threads(int R, int RH, int RW) {
   int nseg = (RH + R - 1) / R;
   int t = nseg * RW;
@@ -1004,7 +1032,7 @@ static hipError_t cs_adj_dispatch(const AdjArgs &a, int nblocks, int nthreads, s
 hipError_t launch_fwd_cs(int nt, int R, int nl, bool rec, const FwdArgs &a, int RHmax,
                          int RWmax, hipStream_t s) {
   int nblocks = a.B * a.ntiles;
-  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax);
+  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax, true);
   int nthreads = cs_threads(R, RHmax, RWmax);
   if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt) return hipErrorInvalidConfiguration;
 #define X(A, B) \
@@ -1016,7 +1044,7 @@ hipError_t launch_fwd_cs(int nt, int R, int nl, bool rec, const FwdArgs &a, int 
 
 hipError_t launch_adj_cs(int nt, int R, const AdjArgs &a, int RHmax, int RWmax, hipStream_t s) {
   int nblocks = a.B * a.ntiles;
-  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax);
+  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax, false);
   int nthreads = cs_threads(R, RHmax, RWmax);
   if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt) return hipErrorInvalidConfiguration;
 #define X(A, B) if (nt == A && R == B) return cs_adj_dispatch<A, B>(a, nblocks, nthreads, shmem, s);
@@ -1030,9 +1058,8 @@ hipError_t launch_adj_cs(int nt, int R, const AdjArgs &a, int RHmax, int RWmax, 
 bool cs_nl_variant_exists(int nt, int r) { return nt == 1024 && r == 4; }
 
 size_t cs_nl_shmem_bytes(int R, int RH, int RW) {
-  int nseg = (RH + R - 1) / R;
-  int CS = (nseg * R + 2) | 1;
-  return cs_shmem_bytes(R, RH, RW) + 16 + (size_t)(RW + 2) * CS * 16;
+  int CS = cs_stride_alloc(R, RH, false);
+  return cs_shmem_bytes(R, RH, RW, false) + 16 + (size_t)(RW + 2) * CS * 16;
 }
 
 hipError_t launch_adj_cs_nl(int nt, int R, bool gb, const AdjArgs &a, const NlCarry &cy,
@@ -1088,7 +1115,7 @@ hipError_t launch_fwd_cs_batch(int nt, int R, bool rec, const BatchFwd &bl, int 
                                int RWmax, hipStream_t s) {
   int nblocks = bl.start[bl.ngrp];
   if (nblocks <= 0) return hipSuccess;
-  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax);
+  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax, true);
   int nthreads = cs_threads(R, RHmax, RWmax);
   if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt) return hipErrorInvalidConfiguration;
 #define X(A, B) \
@@ -1102,7 +1129,7 @@ hipError_t launch_adj_cs_batch(int nt, int R, const BatchAdj &bl, int RHmax, int
                                hipStream_t s) {
   int nblocks = bl.start[bl.ngrp];
   if (nblocks <= 0) return hipSuccess;
-  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax);
+  size_t shmem = cs_shmem_bytes(R, RHmax, RWmax, false);
   int nthreads = cs_threads(R, RHmax, RWmax);
   if (shmem + 1024 > (size_t)cs_max_dyn_lds() || nthreads > nt) return hipErrorInvalidConfiguration;
 #define X(A, B) \

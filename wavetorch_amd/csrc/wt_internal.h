@@ -28,8 +28,54 @@ constexpr int TAP_W = 4;
 //   [6] o_c0 [7] o_c1  output cols  [o_c0, o_c1)
 //   [8] sp_lo [9] sp_hi  special pixels with x in region rows
 //   [10] tp_lo [11] tp_hi taps with x in output rows
-//   [12..15] unused
+//   [12] RWm [13] CS   column-segment kernels: thread-mapping width and LDS
+//                      column stride (cs_layout; 0 = RW and the shortest)
+//   [14..15] unused
 constexpr int TILE_W = 16;
+
+// LDS bank mapping of the field buffers.  A wave's ds_read_b32 / ds_write_b32
+// is served 32 lanes per cycle from 32 four-byte banks (bank = dword index
+// mod 32; rocprofv3's SQ_LDS_BANK_CONFLICT agrees on these kernels: a region
+// 96 columns wide counts ~0.07 conflict cycles per LDS instruction, one 107
+// wide ~0.9).  Thread t owns (segment k, column c) = (t / RWm, t % RWm),
+// whose cells start at LDS index (c+1)*CS + k*R + 1.  The 32 lanes of a
+// half-wave hold consecutive columns except at one wrap from (k, RWm-1) to
+// (k+1, 0); counting the lanes after the wrap as columns RWm, RWm+1, ... of
+// segment k moves their index by R - RWm*CS.  With CS odd and
+// RWm*CS == R (mod 32) that move is a multiple of 32 words, so the 32 lanes
+// fall in 32 distinct banks (the 16-byte material reads of the nonlinear
+// adjoint, 8 lanes per cycle, then too); a multiple of 32 for RWm puts every
+// wrap between half-waves.  cs_layout takes the smallest RWm >= RW (columns
+// RW..RWm-1 idle) and then the smallest CS >= the shortest odd stride that
+// satisfy either within `maxthr` threads and cs_lds_slack(R) extra words per
+// column; failing that, RWm = RW and the shortest stride.
+__host__ __device__ inline int cs_lds_slack(int R) {
+  int v = 0;  // 2-adic valuation of R: CS is then fixed modulo 32 >> v
+  while (v < 4 && ((R >> v) & 1) == 0) ++v;
+  const int s = (32 >> v) - 2;
+  return s < 14 ? s : 14;
+}
+__host__ __device__ inline void cs_layout(int RH, int RW, int R, int maxthr, int &RWm, int &CS) {
+  const int nseg = (RH + R - 1) / R;
+  const int cs0 = (nseg * R + 2) | 1;
+  const int slack = cs_lds_slack(R);
+  RWm = RW;
+  CS = cs0;
+  for (int d = 0; d < 16; ++d) {
+    const int w = RW + d;
+    if (w * nseg > maxthr) return;
+    if ((w & 31) == 0) {  // every wrap falls between half-waves
+      RWm = w;
+      return;
+    }
+    for (int e = 0; e <= slack; e += 2)
+      if (((w * (cs0 + e) - R) & 31) == 0) {
+        RWm = w;
+        CS = cs0 + e;
+        return;
+      }
+  }
+}
 
 // Arguments of the fused (temporally blocked) forward kernel.
 struct FwdArgs {

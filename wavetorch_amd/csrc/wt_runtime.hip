@@ -75,7 +75,7 @@ hipError_t launch_adj_tiled(int nt, int cpt, bool unit_h, const AdjArgs &a, int 
 bool tiled_variant_exists(int nt, int cpt);
 size_t tiled_shmem_bytes(int nt, int cpt, int LW);
 bool cs_variant_exists(int nt, int r);
-size_t cs_shmem_bytes(int R, int RH, int RW);
+size_t cs_shmem_bytes(int R, int RH, int RW, bool padded);
 int cs_threads(int R, int RH, int RW);
 hipError_t launch_fwd_cs(int nt, int R, int nl, bool rec, const FwdArgs &a, int RHmax,
                          int RWmax, hipStream_t s);
@@ -275,13 +275,14 @@ static bool tiles_ok(const Plan &p, int nt, int cpt) {
 
 // Column-segment family: block of cs_threads() threads, R rows per thread;
 // per thread at most 2 special rows (adjoint) and source multiplicity <= 2.
-static bool tiles_ok_cs(const Plan &p, int nt, int R) {
+// `fwd`: forward tiles (their LDS allocation holds the padded layout).
+static bool tiles_ok_cs(const Plan &p, int nt, int R, bool fwd = true) {
   if (!cs_variant_exists(nt, R)) return false;
   for (int ti = 0; ti < p.ntiles; ++ti) {  // tables staged in LDS per tile
     const int32_t *t = &p.h_tiles[ti * TILE_W];
     if (t[9] - t[8] > 256 || t[11] - t[10] > 256) return false;
   }
-  if (cs_shmem_bytes(R, p.RHmax, p.RWmax) + 1024 > LDS_LIMIT) return false;
+  if (cs_shmem_bytes(R, p.RHmax, p.RWmax, fwd) + 1024 > LDS_LIMIT) return false;
   int nthr = cs_threads(R, p.RHmax, p.RWmax);
   if (nthr > nt) return false;
   for (int ti = 0; ti < p.ntiles; ++ti) {
@@ -312,10 +313,10 @@ static const int kCsVariants[][2] = {{1024, 4},  {1024, 6},  {1024, 8},  {1024, 
 
 // A 1024-thread variant with at most maxR rows per thread that holds the
 // current tiling (the smallest such R).
-static bool fit_cs(const Plan &p, int maxR, int &R) {
+static bool fit_cs(const Plan &p, int maxR, int &R, bool fwd = true) {
   for (auto &v : kCsVariants) {
     if (v[0] != 1024 || v[1] > maxR) continue;
-    if (tiles_ok_cs(p, v[0], v[1])) {
+    if (tiles_ok_cs(p, v[0], v[1], fwd)) {
       R = v[1];
       return true;
     }
@@ -324,9 +325,9 @@ static bool fit_cs(const Plan &p, int maxR, int &R) {
 }
 
 // Smallest R whose block fits (fewest rows per thread -> most threads).
-static bool pick_variant_cs(Plan &p, int &nt, int &R) {
+static bool pick_variant_cs(Plan &p, int &nt, int &R, bool fwd = true) {
   for (auto &v : kCsVariants) {
-    if (tiles_ok_cs(p, v[0], v[1])) {
+    if (tiles_ok_cs(p, v[0], v[1], fwd)) {
       nt = v[0];
       R = v[1];
       return true;
@@ -715,7 +716,7 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
         if (d <= 0 || d > p->K || (d != whole && d > 1 && own < 2 * d)) continue;
         build_tiles(*p, as0, ac0, d);
         int r = 0;
-        if (fit_cs(*p, 10, r)) {
+        if (fit_cs(*p, 10, r, false)) {
           atb = d;
           break;
         }
@@ -736,7 +737,7 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
       acp = 4;
       auto fits = [&](int q, int qc, int d) {
         build_tiles(*p, q, qc, d);
-        return tiles_ok_cs(*p, ant, acp) &&
+        return tiles_ok_cs(*p, ant, acp, false) &&
                cs_nl_shmem_bytes(acp, p->RHmax, p->RWmax) + 1024 <= LDS_LIMIT;
       };
       const bool fixed = tun.a_strips > 0;
@@ -766,8 +767,8 @@ static int plan_build(const wt_config *cfg, int32_t B, int32_t T, float cmax_geo
       build_tiles(*p, as, ac, atb);
       // an explicit variant (adjoint's own, or the forward's) is tried as is;
       // otherwise the fewest rows per thread that fit
-      if (tun.a_nt > 0 || (tuning && tuning->nt > 0)) aok = tiles_ok_cs(*p, ant, acp);
-      if (!aok && tun.a_nt <= 0) aok = pick_variant_cs(*p, ant, acp);
+      if (tun.a_nt > 0 || (tuning && tuning->nt > 0)) aok = tiles_ok_cs(*p, ant, acp, false);
+      if (!aok && tun.a_nt <= 0) aok = pick_variant_cs(*p, ant, acp, false);
     }
     if (!aok) {
       set_error("wt_plan_create: adjoint tuning (strips=%d tcols=%d tblock=%d nt=%d cpt=%d) "
@@ -846,8 +847,28 @@ int wt_plan_create(const wt_config *cfg, int32_t B, int32_t T, float cmax_geom,
 
 }  // extern "C"
 
+// Column-segment thread mapping and LDS stride of every tile (descriptor
+// words 12, 13) for R rows per thread, padding a tile's columns only within
+// the waves it occupies anyway (an extra active wave costs these VALU-bound
+// blocks more than the bank conflicts it removes).
+static void set_tile_layouts(std::vector<int32_t> &tiles, int R, int RHmax, int RWmax) {
+  if (R <= 0) return;
+  const int launch_thr = cs_threads(R, RHmax, RWmax);
+  for (size_t i = 0; i + TILE_W <= tiles.size(); i += TILE_W) {
+    int RWm = 0, CS = 0;
+    const int maxthr = std::min(launch_thr, cs_threads(R, tiles[i + 1], tiles[i + 3]));
+    cs_layout(tiles[i + 1], tiles[i + 3], R, maxthr, RWm, CS);
+    tiles[i + 12] = RWm;
+    tiles[i + 13] = CS;
+  }
+}
+
 // Device copies of a plan's tables (synchronous).
 static int plan_upload(Plan *p) {
+  if (p->tiled && p->tun.kind == 0) {
+    // forward tiles only: the adjoints wait little on LDS (measured: no gain)
+    set_tile_layouts(p->h_tiles, p->tun.cpt, p->RHmax, p->RWmax);
+  }
   auto up = [&](int32_t **dst, const std::vector<int32_t> &v) -> int {
     size_t n = std::max<size_t>(v.size(), 1) * sizeof(int32_t);
     hipError_t e = hipMalloc((void **)dst, n);
