@@ -352,7 +352,7 @@ def run_f64(args, dev):
     """The training step of a "train" workload in float64 (set_dtype('float64'),
     the precision the reference's example configurations train in): geometry
     and loss as torch ops in float64, the double-precision time loop
-    (k64_fwd one launch per step, the adjoint as k64_adj_a + k64_adj_b per
+    (k64_fwd one launch per step, the adjoint as one k64_adj launch per
     step over the stored fields).  HIP events on the launch stream bracket
     the forward sweep and the backward of each timed step; the roofline is
     that of the sweep that takes longer, at its algorithmic bytes (24 B per
@@ -392,7 +392,7 @@ def run_f64(args, dev):
             bwd_ms += ev[1].elapsed_time(ev[2]) / args.steps
         log(f"[bench] float64 sweeps: forward {fwd_ms:.2f} ms, backward {bwd_ms:.2f} ms")
         fwd = fwd_ms >= bwd_ms
-        kname = "k64_fwd" if fwd else "k64_adj_a + k64_adj_b (one adjoint step)"
+        kname = "k64_fwd" if fwd else "k64_adj"
         avg_s = (fwd_ms if fwd else bwd_ms) / 1e3 / T
         per_launch = B * Nx * Ny
         byts = (24.0 if fwd else 32.0) * per_launch
@@ -400,7 +400,7 @@ def run_f64(args, dev):
                         "time", "achieved": round(byts / avg_s / 1e9, 1), "unit": "GB/s",
                "frac_of_hbm_peak": round(byts / avg_s / PEAK_HBM, 4)}
         # counter bytes (profiles/traffic_<workload>_f64.json, --summarize-pmc)
-        knames = ["wt::k64_fwd"] if fwd else ["wt::k64_adj_a", "wt::k64_adj_b"]
+        knames = ["wt::k64_fwd"] if fwd else ["wt::k64_adj"]
         ks = [kernel_counters(f"{args.workload}_f64", k, "float64") for k in knames]
         traffic_b = (sum(k["hbm_bytes_per_launch"] for k in ks)
                      if all(k.get("hbm_bytes_per_launch") for k in ks) else None)

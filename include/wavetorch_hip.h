@@ -240,7 +240,7 @@ typedef struct wt_scalars_f64 {
  *   saved (with_grad): every field u_n, n = -1..T, ((T+2)*B*Nx*Ny doubles,
  *          what the reference's autograd tape keeps; no checkpoint recompute)
  *   temp:  three fields (forward without gradients), or the adjoint's
- *          lambdas, coefficient products and gradient accumulators. */
+ *          three lambdas, gradient accumulators and lambda at the sources. */
 int wt_workspace_bytes_f64(const void *plan, int32_t with_grad, int64_t *saved,
                            int64_t *temp);
 

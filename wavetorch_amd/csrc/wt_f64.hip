@@ -10,10 +10,10 @@
 //                cell.py:12-17 in the reference's operation order (with the
 //                nonlinear b and c of cell.py:94-102), sources (source.py:15-22)
 //                and probes (probe.py:14-27), compiled without contraction
-//   k64_adj_a    adjoint iteration m, first half: step m's coefficients at
-//                u_m give w_m * lambda_{m+1} and the pointwise term
-//                (s2 A + dv/db db/du + dv/dc dc/du) * lambda_{m+1}
-//   k64_adj_b    second half: lambda_m = that term + L(w lambda_{m+1})
+//   k64_adj      adjoint iteration m: step m's coefficients at u_m give the
+//                pointwise term (s2 A + dv/db db/du + dv/dc dc/du) *
+//                lambda_{m+1} and, staged with a halo in LDS, w_m * lambda_{m+1};
+//                lambda_m = that term + L(w lambda_{m+1})
 //                - (A Bm)_{m+1} lambda_{m+2} + probe seeds, then step m-1's
 //                parameter gradients (dJ/dc_linear, dJ/db, dJ/drho) and
 //                lambda_m at source pixels (dJ/dx)
@@ -125,60 +125,104 @@ __global__ void k64_fwd(F64Args a, const double *__restrict__ u1, const double *
   }
 }
 
-// Adjoint iteration m, first half (step m at u_m, u_{m-1}).
-__global__ void k64_adj_a(F64Args a, const double *__restrict__ um,
-                          const double *__restrict__ um1, const double *__restrict__ L1,
-                          double *__restrict__ wl, double *__restrict__ lp) {
-  const int Nx = a.Nx, Ny = a.Ny;
-  const int64_t F = (int64_t)Nx * Ny;
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= F * a.B) return;
-  const int b = (int)(gid / F);
-  const int64_t q = gid - (int64_t)b * F;
-  const int i = (int)(q / Ny), j = (int)(q - (int64_t)i * Ny);
-  const double *u = um + (int64_t)b * F;
-  const double uu = u[q];
-  const double lap = lap64(u, q, i, j, Nx, Ny, a.t1, a.t4);
-  double bb, cc, sd, c2raw;
-  bool clamped;
-  mat64(a, q, uu, bb, cc, sd, c2raw, clamped);
-  double dbdu = 0.0, dcdu = 0.0;
-  if (a.nlf & 1) {
-    const double r = uu / a.sat_uth, d = 1.0 + r * r;
-    dbdu = a.rho[q] * a.sat_b0 * (-2.0 * r / a.sat_uth) / (d * d);
-  }
-  if ((a.nlf & 2) && !clamped) dcdu = 2.0 * a.rho[q] * a.c_nl * uu;
-  const double bd = bb * a.dtm1;
-  const double A = 1.0 / (a.dtm2 + bd), Bm = a.dtm2 - bd;
-  const double v = A * ((a.s2 * uu - Bm * um1[gid]) + (cc * cc) * lap);
-  const double dvdb = A * a.dtm1 * (um1[gid] - v);
-  const double dvdc = A * 2.0 * cc * lap;
-  const double l1 = L1[gid];
-  wl[gid] = A * (cc * cc) * l1;
-  lp[gid] = (a.s2 * A + dvdb * dbdu + dvdc * dcdu) * l1;
-}
+// Adjoint iteration m in one launch: lambda_m, then step m-1's gradients.
+// A workgroup owns a tile of K64_TX rows x K64_TY columns of one sample and
+// stages w_m lambda_{m+1} (w = A c^2 at u_m, pointwise) on the tile and a
+// one-cell halo in LDS, so L(w lambda_{m+1}) needs no second launch.
+//   lambda_m = (s2 A + dv/db db/du + dv/dc dc/du) lambda_{m+1}
+//              + L(w lambda_{m+1}) - (A Bm)_{m+1} lambda_{m+2}
+//              + probe seeds + field seeds
+// (A Bm)_{m+1}, the factor of u_m in step m+1, is re-formed from u_{m+1}
+// (saturable damping) or from b alone; `up1` is NULL when step m+1 does not
+// exist (m >= T-1, where lambda_{m+2} = 0).
+constexpr int K64_TX = 16, K64_TY = 32, K64_NT = K64_TX * K64_TY;
+constexpr int K64_HW = K64_TY + 2, K64_HN = (K64_TX + 2) * K64_HW;
 
-// Adjoint iteration m, second half: lambda_m, then step m-1's gradients.
-// a2 holds A*Bm of a step at (b, cell) in a ring of two slots by step parity:
-// this iteration reads step m+1's factor from the slot it then overwrites with
-// step m-1's (same parity), each thread its own cell.
-__global__ void k64_adj_b(F64Args a, const double *__restrict__ wl,
-                          const double *__restrict__ lp, const double *__restrict__ L2,
-                          double *__restrict__ a2, const double *__restrict__ um,
-                          const double *__restrict__ umm1, const double *__restrict__ umm2,
-                          const double *__restrict__ ybar, const double *__restrict__ fbar,
-                          double *__restrict__ L0, double *__restrict__ gacc_c,
-                          double *__restrict__ gacc_b, double *__restrict__ gacc_r,
-                          double *__restrict__ gsrc, int m) {
+__global__ __launch_bounds__(K64_NT) void k64_adj(
+    F64Args a, int tiles_x, int tiles_y, const double *__restrict__ up1,
+    const double *__restrict__ um, const double *__restrict__ umm1,
+    const double *__restrict__ umm2, const double *__restrict__ L1,
+    const double *__restrict__ L2, const double *__restrict__ ybar,
+    const double *__restrict__ fbar, double *__restrict__ L0, double *__restrict__ gacc_c,
+    double *__restrict__ gacc_b, double *__restrict__ gacc_r, double *__restrict__ gsrc,
+    int m) {
+  __shared__ double wls[K64_HN];
   const int Nx = a.Nx, Ny = a.Ny;
   const int64_t F = (int64_t)Nx * Ny;
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= F * a.B) return;
-  const int b = (int)(gid / F);
-  const int64_t q = gid - (int64_t)b * F;
-  const int i = (int)(q / Ny), j = (int)(q - (int64_t)i * Ny);
-  double lam = lp[gid] + lap64(wl + (int64_t)b * F, q, i, j, Nx, Ny, a.t1, a.t4);
-  lam = lam - a2[gid] * L2[gid];
+  // neighbouring tiles of a sample on one XCD: their halo lines meet in its L2
+  int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int tcol = blk % tiles_y;
+  blk /= tiles_y;
+  const int trow = blk % tiles_x;
+  const int b = blk / tiles_x;
+  const int i0 = trow * K64_TX, j0 = tcol * K64_TY;
+  const int64_t boff = (int64_t)b * F;
+  // w_m lambda_{m+1} on the tile and its halo (zero outside the domain)
+  for (int p = threadIdx.x; p < K64_HN; p += K64_NT) {
+    const int pi = p / K64_HW, pj = p - pi * K64_HW;
+    const int i = i0 - 1 + pi, j = j0 - 1 + pj;
+    double w = 0.0;
+    if (i >= 0 && i < Nx && j >= 0 && j < Ny) {
+      const int64_t q = (int64_t)i * Ny + j;
+      double bb, cc, sd, c2raw;
+      bool clamped;
+      mat64(a, q, um[boff + q], bb, cc, sd, c2raw, clamped);
+      const double bd = bb * a.dtm1;
+      const double A = 1.0 / (a.dtm2 + bd);
+      w = A * (cc * cc) * L1[boff + q];
+    }
+    wls[p] = w;
+  }
+  __syncthreads();
+  const int li = threadIdx.x / K64_TY, lj = threadIdx.x - li * K64_TY;
+  const int i = i0 + li, j = j0 + lj;
+  if (i >= Nx || j >= Ny) return;
+  const int64_t q = (int64_t)i * Ny + j, gid = boff + q;
+  // step m at u_m, u_{m-1}: the pointwise term
+  double lam;
+  {
+    const double *u = um + boff;
+    const double uu = u[q];
+    const double lap = lap64(u, q, i, j, Nx, Ny, a.t1, a.t4);
+    double bb, cc, sd, c2raw;
+    bool clamped;
+    mat64(a, q, uu, bb, cc, sd, c2raw, clamped);
+    double dbdu = 0.0, dcdu = 0.0;
+    if (a.nlf & 1) {
+      const double r = uu / a.sat_uth, d = 1.0 + r * r;
+      dbdu = a.rho[q] * a.sat_b0 * (-2.0 * r / a.sat_uth) / (d * d);
+    }
+    if ((a.nlf & 2) && !clamped) dcdu = 2.0 * a.rho[q] * a.c_nl * uu;
+    const double bd = bb * a.dtm1;
+    const double A = 1.0 / (a.dtm2 + bd), Bm = a.dtm2 - bd;
+    const double um1 = umm1[gid];
+    const double v = A * ((a.s2 * uu - Bm * um1) + (cc * cc) * lap);
+    const double dvdb = A * a.dtm1 * (um1 - v);
+    const double dvdc = A * 2.0 * cc * lap;
+    lam = (a.s2 * A + dvdb * dbdu + dvdc * dcdu) * L1[gid];
+  }
+  // + L(w lambda_{m+1}), the Laplacian's order of lap64 on the staged values
+  {
+    const int c0 = (li + 1) * K64_HW + (lj + 1);
+    double s = a.t1 * wls[c0 - K64_HW];
+    s = s + a.t1 * wls[c0 - 1];
+    s = s + a.t4 * wls[c0];
+    s = s + a.t1 * wls[c0 + 1];
+    lam = lam + (s + a.t1 * wls[c0 + K64_HW]);
+  }
+  if (up1) {  // - (A Bm)_{m+1} lambda_{m+2}
+    double bb = a.b[q];
+    if (a.nlf & 1) {
+      double r = up1[gid] / a.sat_uth;
+      r = fabs(r);
+      r = r * r;
+      r = r + 1.0;
+      bb = bb + a.rho[q] * (a.sat_b0 / r);
+    }
+    const double bd = bb * a.dtm1;
+    const double A = 1.0 / (a.dtm2 + bd), Bm = a.dtm2 - bd;
+    lam = lam - (A * Bm) * L2[gid];
+  }
   const int sp = a.spmap[q];
   if (sp >= 0 && ybar) {
     // direct seeds: y[m-1] reads u_m (probe.py:14-27)
@@ -192,7 +236,7 @@ __global__ void k64_adj_b(F64Args a, const double *__restrict__ wl,
   if (fbar) lam += fbar[((int64_t)b * a.T + (m - 1)) * F + q];
   L0[gid] = lam;
   // step m-1 produced u_m = v_m + sources from (u_{m-1}, u_{m-2})
-  const double *u = umm1 + (int64_t)b * F;
+  const double *u = umm1 + boff;
   const double uu = u[q];
   const double lap = lap64(u, q, i, j, Nx, Ny, a.t1, a.t4);
   double bb, cc, sd, c2raw;
@@ -206,7 +250,6 @@ __global__ void k64_adj_b(F64Args a, const double *__restrict__ wl,
   gacc_c[gid] += dvdc * lam;
   if (gacc_b) gacc_b[gid] += dvdb * lam;
   if (gacc_r) gacc_r[gid] += (dvdb * sd + dvdc * c2raw) * lam;
-  a2[gid] = A * Bm;
   if (sp >= 0) {
     const int32_t *e = a.spx + sp * SPX_W;
     if (e[2] > 0) gsrc[((int64_t)b * a.T + (m - 1)) * a.nslot + e[3]] = lam;
@@ -251,8 +294,8 @@ static hipError_t fill64(double *p, int64_t n, double v, hipStream_t s) {
 
 // Workspace layout (doubles).  saved: the tape U[n + 1] = u_n, n = -1..T.
 // temp (forward without gradients): three rotating fields; (adjoint): three
-// lambdas, w*lambda, the pointwise term, the A*Bm ring (2), three gradient
-// accumulators, and lambda at source slots (B, T, nslot).
+// lambdas, three gradient accumulators, and lambda at source slots
+// (B, T, nslot).
 struct Ws64 {
   int64_t BF, tape, fwd_temp, adj_temp;
 };
@@ -261,7 +304,7 @@ static Ws64 ws64(const PlanView &v) {
   w.BF = (int64_t)v.B * v.Nx * v.Ny;
   w.tape = (int64_t)(v.T + 2) * w.BF;
   w.fwd_temp = 3 * w.BF;
-  w.adj_temp = 10 * w.BF + (int64_t)v.B * v.T * std::max(v.nslot, 1);
+  w.adj_temp = 6 * w.BF + (int64_t)v.B * v.T * std::max(v.nslot, 1);
   return w;
 }
 
@@ -365,19 +408,18 @@ extern "C" int wt_loop_adj_f64(void *plan, const wt_scalars_f64 *sc, const doubl
   auto Un = [&](int n) { return U + (int64_t)(n + 1) * BF; };  // u_n, n >= -1
   double *t = (double *)temp;
   double *lam[3] = {t, t + BF, t + 2 * BF};
-  double *wl = t + 3 * BF, *lp = t + 4 * BF, *a2 = t + 5 * BF;  // a2: 2 slots
-  double *gc = t + 7 * BF, *gb = t + 8 * BF, *gr = t + 9 * BF;
-  double *gsrc = t + 10 * BF;
+  double *gc = t + 3 * BF, *gb = t + 4 * BF, *gr = t + 5 * BF;
+  double *gsrc = t + 6 * BF;
   WT_HIP(fill64(t, w.adj_temp, 0.0, s));  // lambda_{T+1} = lambda_{T+2} = 0
-  const unsigned g = nblk64(BF);
+  const int tiles_x = (v.Nx + K64_TX - 1) / K64_TX, tiles_y = (v.Ny + K64_TY - 1) / K64_TY;
+  const unsigned g = (unsigned)((int64_t)v.B * tiles_x * tiles_y);
   double *l1 = lam[0], *l2 = lam[1], *l0 = lam[2];
   for (int m = v.T; m >= 1; --m) {
-    hipLaunchKernelGGL(k64_adj_a, dim3(g), dim3(256), 0, s, a, Un(m), Un(m - 1), l1, wl, lp);
-    WT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k64_adj_b, dim3(g), dim3(256), 0, s, a, wl, lp, l2,
-                       a2 + (int64_t)((m + 1) & 1) * BF, Un(m), Un(m - 1), Un(m - 2), ybar,
-                       fbar, l0, gc, grad_b ? gb : nullptr, (v.nlf & 3) ? gr : nullptr, gsrc,
-                       m);
+    // step m+1 (factor of u_m in u_{m+2}) exists for m + 1 <= T - 1
+    const double *up1 = m + 1 <= v.T - 1 ? Un(m + 1) : nullptr;
+    hipLaunchKernelGGL(k64_adj, dim3(g), dim3(K64_NT), 0, s, a, tiles_x, tiles_y, up1, Un(m),
+                       Un(m - 1), Un(m - 2), l1, l2, ybar, fbar, l0, gc,
+                       grad_b ? gb : nullptr, (v.nlf & 3) ? gr : nullptr, gsrc, m);
     WT_HIP(hipGetLastError());
     double *tmp = l2; l2 = l1; l1 = l0; l0 = tmp;  // (l2, l1) <- (l1, l0)
   }
