@@ -18,6 +18,8 @@ the reference's own example configurations train in float64).
   (oracle/ref_torch.py, on the CPU).
 * TimeStep.apply in float64 (the single-step API): output within 1e-14 and
   gradients within 1e-12 of the restated reference step.
+* WaveGeometryHoley in float64: c and its hole-parameter gradients against
+  the restated reference in float64.
 """
 import ctypes
 
@@ -173,3 +175,32 @@ def test_timestep_float64(cuda):
     assert rel(out.detach().cpu().numpy(), ref.detach().numpy()) < 1e-14
     for a, r in zip(dins, ins):
         assert rel(a.grad.cpu().numpy(), r.grad.numpy()) < 1e-12
+
+
+def test_holey_geometry_float64(cuda):
+    """WaveGeometryHoley in float64 (geom.py:88-135): c and dJ/d(x, y, r) as
+    the reference's torch ops on the device equal the restated reference
+    (oracle/ref_torch.holey_c) in float64 on the CPU, off-grid hole centres."""
+    set_dtype("float64")
+    try:
+        from wavetorch_amd import WaveGeometryHoley
+        rng = np.random.default_rng(21)
+        shape = (48, 40)
+        x = rng.uniform(8, 40, 6) + 0.37
+        y = rng.uniform(8, 32, 6) + 0.29
+        r = rng.uniform(2.0, 5.0, 6)
+        g = WaveGeometryHoley(shape, 1.0, c0=1.0, c1=0.5, abs_N=5, eta=0.5, beta=100.0,
+                              x=x, y=y, r=r).to(cuda)
+        w = torch.from_numpy(rng.standard_normal(shape))
+        c = g.c
+        (c * w.to(cuda)).sum().backward()
+        torch.cuda.synchronize()
+        hx, hy, hr = (torch.from_numpy(v).requires_grad_(True) for v in (x, y, r))
+        c_r = ref_torch.holey_c(hx, hy, hr, shape, 0.5, 100.0, 1.0, 0.5)
+        (c_r * w).sum().backward()
+        assert c.dtype == torch.float64
+        assert rel(c.detach().cpu().numpy(), c_r.detach().numpy()) < 1e-13
+        for p, q in ((g.x, hx), (g.y, hy), (g.r, hr)):
+            assert rel(p.grad.cpu().numpy(), q.grad.numpy()) < 1e-10
+    finally:
+        set_dtype("float32")
