@@ -27,8 +27,8 @@ def _check_dev(*ts):
             raise RuntimeError("wavetorch_amd: the time step runs on the HIP device only; "
                                "move the tensors/model with .to('cuda')")
         if t.dtype != torch.float32:
-            raise NotImplementedError(f"wavetorch_amd: the time step computes in float32 "
-                                      f"(got {t.dtype})")
+            raise NotImplementedError(f"wavetorch_amd: the time step computes in float32 or "
+                                      f"float64 (got {t.dtype})")
 
 
 def _time_step_float64(b, c, y1, y2, dt, h):

@@ -279,7 +279,8 @@ def _require_hip(t, what):
         raise RuntimeError(
             f"wavetorch_amd: {what} runs on the HIP device only; move the model with .to('cuda')")
     if t.dtype != torch.float32:
-        raise NotImplementedError(f"wavetorch_amd: {what} computes in float32 (got {t.dtype})")
+        raise NotImplementedError(f"wavetorch_amd: {what} computes in float32 or float64 "
+                                  f"(got {t.dtype})")
 
 
 class WaveGeometryFreeForm(WaveGeometry):
