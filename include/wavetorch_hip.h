@@ -76,6 +76,18 @@ int wt_coef_linear(const float *b, const float *c, int64_t n, float dt,
                    float *A, float *Bm, float *C2, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Laplacian: backs operators._laplacian (operators.py:5-11)                  */
+/* ------------------------------------------------------------------------ */
+
+/* out = h^-2 * (5-point stencil of u) with zero padding, u and out
+ * (B,Nx,Ny), out not aliasing u; in conv2d's accumulation order, so bitwise
+ * equal to the reference's fp32 CPU result for any h.  The operator is its
+ * own adjoint (symmetric stencil, zero padding): the backward is the same
+ * call on the output gradient. */
+int wt_laplacian(const float *u, float *out, int32_t B, int32_t Nx, int32_t Ny, float h,
+                 void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Single step: backs cell.TimeStep (cell.py:20-44)                          */
 /* ------------------------------------------------------------------------ */
 

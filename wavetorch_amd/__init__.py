@@ -7,7 +7,7 @@ the geometry parameterisation run in hand-written HIP kernels loaded from
 ``wavetorch_amd/_lib/libwavetorch_hip.so`` through the C ABI declared in
 ``include/wavetorch_hip.h``.
 """
-from . import cell, data, geom, io, loss, probe, rnn, source, utils
+from . import cell, data, geom, io, loss, operators, probe, rnn, source, utils
 from .cell import TimeStep, WaveCell
 from .geom import WaveGeometryFreeForm, WaveGeometryHoley
 from .probe import WaveIntensityProbe, WaveProbe

@@ -40,6 +40,8 @@ int hip_fail(hipError_t e, const char *what) {
 // launchers implemented in wt_plain.hip / wt_tiled.hip / wt_geom.hip
 hipError_t launch_coef_linear(const float *b, const float *c, int64_t n, float dtm1, float dtm2,
                               float *A, float *Bm, float *C2, hipStream_t s);
+hipError_t launch_laplacian(const float *u, int B, int Nx, int Ny, float t1, float t4,
+                            float *out, hipStream_t s);
 hipError_t launch_step_fwd(bool unit_h, const float *b, const float *c, int bc_ps,
                            const float *y1, const float *y2, int B, int Nx, int Ny, float dtm1,
                            float dtm2, float s2, float t1, float t4, float *y, hipStream_t s);
@@ -442,6 +444,18 @@ int wt_coef_linear(const float *b, const float *c, int64_t n, float dt, float *A
   float dtm1, dtm2, s2, t1, t4;
   scalars(dt, 1.0f, dtm1, dtm2, s2, t1, t4);
   WT_HIP(launch_coef_linear(b, c, n, dtm1, dtm2, A, Bm, C2, (hipStream_t)stream));
+  return WT_OK;
+}
+
+int wt_laplacian(const float *u, float *out, int32_t B, int32_t Nx, int32_t Ny, float h,
+                 void *stream) {
+  if (!u || !out || u == out || B < 0 || Nx <= 0 || Ny <= 0 || !(h > 0) || !std::isfinite(h)) {
+    set_error("wt_laplacian: invalid argument");
+    return WT_EINVAL;
+  }
+  float dtm1, dtm2, s2, t1, t4;
+  scalars(1.0f, h, dtm1, dtm2, s2, t1, t4);
+  WT_HIP(launch_laplacian(u, B, Nx, Ny, t1, t4, out, (hipStream_t)stream));
   return WT_OK;
 }
 
