@@ -350,7 +350,7 @@ extern "C" int wt_loop_fwd_f64(void *plan, const wt_scalars_f64 *sc, const doubl
                                void *saved, void *temp, void *stream) {
   PlanView v;
   if (!plan || !sc || !b || !c || !x || !temp || (with_grad && !saved) ||
-      plan_view(plan, &v) != WT_OK || (v.nlf && (v.nlf & 3) && !rho)) {
+      plan_view(plan, &v) != WT_OK || ((v.nlf & 3) && !rho)) {
     set_error("wt_loop_fwd_f64: invalid argument");
     return WT_EINVAL;
   }
