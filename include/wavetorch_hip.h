@@ -86,6 +86,9 @@ int wt_coef_linear(const float *b, const float *c, int64_t n, float dt,
  * call on the output gradient. */
 int wt_laplacian(const float *u, float *out, int32_t B, int32_t Nx, int32_t Ny, float h,
                  void *stream);
+/* wt_laplacian in float64 (set_dtype('float64'), utils.py:14-20). */
+int wt_laplacian_f64(const double *u, double *out, int32_t B, int32_t Nx, int32_t Ny,
+                     double h, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Single step: backs cell.TimeStep (cell.py:20-44)                          */
@@ -106,6 +109,16 @@ int wt_step_bwd(const float *b, const float *c, int32_t bc_per_sample,
                 const float *y1, const float *y2, const float *g, int32_t B,
                 int32_t Nx, int32_t Ny, float dt, float h, float *grad_b,
                 float *grad_c, float *grad_y1, float *grad_y2, void *stream);
+
+/* wt_step_fwd / wt_step_bwd in float64 (TimeStep under set_dtype('float64'),
+ * utils.py:14-20): the same arguments, arrays and scalars double. */
+int wt_step_fwd_f64(const double *b, const double *c, int32_t bc_per_sample, const double *y1,
+                    const double *y2, int32_t B, int32_t Nx, int32_t Ny, double dt, double h,
+                    double *y, void *stream);
+int wt_step_bwd_f64(const double *b, const double *c, int32_t bc_per_sample, const double *y1,
+                    const double *y2, const double *g, int32_t B, int32_t Nx, int32_t Ny,
+                    double dt, double h, double *grad_b, double *grad_c, double *grad_y1,
+                    double *grad_y2, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Whole time loop: backs WaveRNN.forward (rnn.py:21-72) and the autograd     */
@@ -372,6 +385,29 @@ int wt_geom_holey_c_bwd(const float *grad_c, const float *raw, const float *x,
 
 /* Bytes of `work` for wt_geom_holey_c_bwd (no counterpart in the reference). */
 int wt_geom_holey_workspace_bytes(int32_t nholes, int32_t Nx, int32_t Ny, int64_t *bytes);
+
+/* The four geometry entry points above in float64 (the geometry of a model
+ * built after set_dtype('float64'), utils.py:14-20): the same arguments,
+ * every array and scalar double; the FreeForm work arrays hold 2*Nx*Ny
+ * doubles, the Holey backward's work wt_geom_holey_workspace_bytes_f64
+ * bytes. */
+int wt_geom_freeform_c_f64(const double *rho, const double *kernel, int32_t radius,
+                           int32_t blur_N, int32_t Nx, int32_t Ny, double eta, double beta,
+                           double c0, double c1, double *c, double *rho_proj, double *blur_out,
+                           double *work, void *stream);
+int wt_geom_freeform_c_bwd_f64(const double *grad_c, const double *blur_in,
+                               const double *kernel, int32_t radius, int32_t blur_N, int32_t Nx,
+                               int32_t Ny, double eta, double beta, double c0, double c1,
+                               double *grad_rho, double *work, void *stream);
+int wt_geom_holey_c_f64(const double *x, const double *y, const double *r, int32_t nholes,
+                        int32_t Nx, int32_t Ny, double eta, double beta, double c0, double c1,
+                        double *c, double *raw_out, void *stream);
+int wt_geom_holey_c_bwd_f64(const double *grad_c, const double *raw, const double *x,
+                            const double *y, const double *r, int32_t nholes, int32_t Nx,
+                            int32_t Ny, double eta, double beta, double c0, double c1,
+                            double *grad_x, double *grad_y, double *grad_r, void *work,
+                            void *stream);
+int wt_geom_holey_workspace_bytes_f64(int32_t nholes, int32_t Nx, int32_t Ny, int64_t *bytes);
 
 /* ------------------------------------------------------------------------ */
 /* Loss head (train.py:61-62 with utils.py:35-36)                            */

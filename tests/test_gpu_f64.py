@@ -11,13 +11,14 @@ the reference's own example configurations train in float64).
   by fp64 central differences and by the dot-product identity in
   test_oracle.py); the sums run in another order, so agreement is to
   rounding, not bitwise.
-* WaveRNN end to end in float64 (FreeForm geometry: blur and projection as
-  the reference's torch ops on the device, the double-precision loop, the
+* WaveRNN end to end in float64 (FreeForm geometry: blur and projection by
+  wt_geom_freeform_c_f64 and its backward, the double-precision loop, the
   reference's CE loss head): probe series within 1e-12 and dJ/drho within
   1e-9 of the reference's op sequence in float64 with its autograd tape
   (oracle/ref_torch.py, on the CPU).
-* TimeStep.apply in float64 (the single-step API): output within 1e-14 and
-  gradients within 1e-12 of the restated reference step.
+* TimeStep.apply in float64 (the single-step API, wt_step_fwd_f64 /
+  wt_step_bwd_f64): output within 1e-14 and gradients within 1e-12 of the
+  restated reference step.
 * WaveGeometryHoley in float64: c and its hole-parameter gradients against
   the restated reference in float64.
 """
@@ -178,8 +179,8 @@ def test_timestep_float64(cuda):
 
 
 def test_holey_geometry_float64(cuda):
-    """WaveGeometryHoley in float64 (geom.py:88-135): c and dJ/d(x, y, r) as
-    the reference's torch ops on the device equal the restated reference
+    """WaveGeometryHoley in float64 (geom.py:88-135): c and dJ/d(x, y, r) from
+    wt_geom_holey_c_f64 / _bwd_f64 equal the restated reference
     (oracle/ref_torch.holey_c) in float64 on the CPU, off-grid hole centres."""
     set_dtype("float64")
     try:

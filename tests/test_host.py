@@ -62,7 +62,10 @@ def test_entry_points_reject_null_arguments():
              "wt_batch_plan_tuning", "wt_batch_workspace_bytes",
              "wt_batch_fwd", "wt_batch_adj", "wt_batch_fwd_adj", "wt_graph_launch",
              "wt_autotune", "wt_loss_power_ce_static", "wt_workspace_bytes_f64",
-             "wt_loop_fwd_f64", "wt_loop_adj_f64"]
+             "wt_loop_fwd_f64", "wt_loop_adj_f64", "wt_laplacian_f64", "wt_step_fwd_f64",
+             "wt_step_bwd_f64", "wt_geom_freeform_c_f64", "wt_geom_freeform_c_bwd_f64",
+             "wt_geom_holey_c_f64", "wt_geom_holey_c_bwd_f64",
+             "wt_geom_holey_workspace_bytes_f64"]
     for n in names:
         argtypes = _lib._SIGS[n][1]
         args = []

@@ -25,6 +25,7 @@ ABI_VERSION = 1
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
 c_f32 = ctypes.c_float
+c_f64 = ctypes.c_double
 c_p = ctypes.c_void_p
 
 
@@ -72,6 +73,11 @@ _SIGS = {
     "wt_check_finite": (ctypes.c_int, [c_p, c_i64, c_p, c_p]),
     "wt_coef_linear": (ctypes.c_int, [c_p, c_p, c_i64, c_f32, c_p, c_p, c_p, c_p]),
     "wt_laplacian": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_f32, c_p]),
+    "wt_laplacian_f64": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_f64, c_p]),
+    "wt_step_fwd_f64": (ctypes.c_int, [c_p, c_p, c_i32, c_p, c_p, c_i32, c_i32, c_i32, c_f64,
+                                       c_f64, c_p, c_p]),
+    "wt_step_bwd_f64": (ctypes.c_int, [c_p, c_p, c_i32, c_p, c_p, c_p, c_i32, c_i32, c_i32,
+                                       c_f64, c_f64, c_p, c_p, c_p, c_p, c_p]),
     "wt_step_fwd": (ctypes.c_int, [c_p, c_p, c_i32, c_p, c_p, c_i32, c_i32, c_i32, c_f32, c_f32,
                                    c_p, c_p]),
     "wt_step_bwd": (ctypes.c_int, [c_p, c_p, c_i32, c_p, c_p, c_p, c_i32, c_i32, c_i32, c_f32,
@@ -99,6 +105,17 @@ _SIGS = {
                                            c_f32, c_f32, c_f32, c_p, c_p, c_p, c_p, c_p]),
     "wt_geom_holey_workspace_bytes": (ctypes.c_int, [c_i32, c_i32, c_i32,
                                                      ctypes.POINTER(c_i64)]),
+    "wt_geom_freeform_c_f64": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_i32, c_f64,
+                                              c_f64, c_f64, c_f64, c_p, c_p, c_p, c_p, c_p]),
+    "wt_geom_freeform_c_bwd_f64": (ctypes.c_int, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_i32,
+                                                  c_f64, c_f64, c_f64, c_f64, c_p, c_p, c_p]),
+    "wt_geom_holey_c_f64": (ctypes.c_int, [c_p, c_p, c_p, c_i32, c_i32, c_i32, c_f64, c_f64,
+                                           c_f64, c_f64, c_p, c_p, c_p]),
+    "wt_geom_holey_c_bwd_f64": (ctypes.c_int, [c_p, c_p, c_p, c_p, c_p, c_i32, c_i32, c_i32,
+                                               c_f64, c_f64, c_f64, c_f64, c_p, c_p, c_p, c_p,
+                                               c_p]),
+    "wt_geom_holey_workspace_bytes_f64": (ctypes.c_int, [c_i32, c_i32, c_i32,
+                                                         ctypes.POINTER(c_i64)]),
     "wt_loss_power_ce": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_p, c_p, c_p, c_p]),
     "wt_loss_power_ce_workspace_bytes": (ctypes.c_int, [c_i32, c_i32, c_i32,
                                                         ctypes.POINTER(c_i64)]),

@@ -26,40 +26,17 @@ def _check_dev(*ts):
         if not t.is_cuda:
             raise RuntimeError("wavetorch_amd: the time step runs on the HIP device only; "
                                "move the tensors/model with .to('cuda')")
-        if t.dtype != torch.float32:
+        if t.dtype not in (torch.float32, torch.float64):
             raise NotImplementedError(f"wavetorch_amd: the time step computes in float32 or "
                                       f"float64 (got {t.dtype})")
-
-
-def _time_step_float64(b, c, y1, y2, dt, h):
-    """The leapfrog step of cell.py:12-17 in float64 as torch ops on the
-    device (5-point Laplacian with zero padding as a conv2d, operators.py:5-11);
-    autograd differentiates it, which equals the reference's hand-written
-    TimeStep.backward (cell.py:26-44)."""
-    for t in (b, c, y1, y2):
-        if not t.is_cuda:
-            raise RuntimeError("wavetorch_amd: the time step runs on the HIP device only; "
-                               "move the tensors/model with .to('cuda')")
-    dt = torch.as_tensor(dt, dtype=torch.float64, device=y1.device)
-    h = torch.as_tensor(h, dtype=torch.float64, device=y1.device)
-    stencil = torch.tensor([[0.0, 1.0, 0.0], [1.0, -4.0, 1.0], [0.0, 1.0, 0.0]],
-                           dtype=torch.float64, device=y1.device)
-    kern = (h ** (-2) * stencil).unsqueeze(0).unsqueeze(0)
-    lap = torch.nn.functional.conv2d(y1.unsqueeze(1), kern, padding=1).squeeze(1)
-    inv = (dt ** -2 + b * dt ** -1).pow(-1)
-    return inv * (2 / dt ** 2 * y1 - (dt ** -2 - b * dt ** -1) * y2 + c.pow(2) * lap)
+    if len({t.dtype for t in ts}) != 1:
+        raise TypeError("wavetorch_amd: b, c, y1 and y2 of a time step must share one dtype")
 
 
 class TimeStep(torch.autograd.Function):
-    """One leapfrog step y = _time_step(b, c, y1, y2, dt, h) (cell.py:12-44).
-    float64 tensors take ``_time_step_float64`` (call ``TimeStep.apply`` as
-    usual)."""
-
-    @staticmethod
-    def apply(b, c, y1, y2, dt, h):  # noqa: D102 (the reference's call form)
-        if y1.dtype == torch.float64:
-            return _time_step_float64(b, c, y1, y2, dt, h)
-        return super(TimeStep, TimeStep).apply(b, c, y1, y2, dt, h)
+    """One leapfrog step y = _time_step(b, c, y1, y2, dt, h) (cell.py:12-44),
+    in float32 (wt_step_fwd / wt_step_bwd) or float64 (the _f64 entry
+    points)."""
 
     @staticmethod
     def forward(ctx, b, c, y1, y2, dt, h):
@@ -75,10 +52,10 @@ class TimeStep(torch.autograd.Function):
         bc, cc = b.contiguous(), c.contiguous()
         y = torch.empty_like(y1c)
         dtf, hf = float(dt), float(h)
-        _lib.check(_lib.lib().wt_step_fwd(bc.data_ptr(), cc.data_ptr(), per_sample,
-                                           y1c.data_ptr(), y2c.data_ptr(), B, Nx, Ny, dtf, hf,
-                                           y.data_ptr(), _lib.stream_ptr(y.device)),
-                   "wt_step_fwd")
+        name = "wt_step_fwd_f64" if y.dtype == torch.float64 else "wt_step_fwd"
+        _lib.check(getattr(_lib.lib(), name)(bc.data_ptr(), cc.data_ptr(), per_sample,
+                                             y1c.data_ptr(), y2c.data_ptr(), B, Nx, Ny, dtf, hf,
+                                             y.data_ptr(), _lib.stream_ptr(y.device)), name)
         ctx.save_for_backward(bc, cc, y1c, y2c)
         ctx.meta = (per_sample, dtf, hf, shapes)
         return y
@@ -87,7 +64,7 @@ class TimeStep(torch.autograd.Function):
     def backward(ctx, grad_output):
         b, c, y1, y2 = ctx.saved_tensors
         per_sample, dt, h, shapes = ctx.meta
-        g = grad_output.contiguous()
+        g = grad_output.to(y1.dtype).contiguous()
         B = y1.shape[0]
         Nx, Ny = y1.shape[-2:]
         need = ctx.needs_input_grad
@@ -95,11 +72,12 @@ class TimeStep(torch.autograd.Function):
         gc = torch.empty_like(c) if need[1] else None
         gy1 = torch.empty_like(y1) if need[2] else None
         gy2 = torch.empty_like(y2) if need[3] else None
-        _lib.check(_lib.lib().wt_step_bwd(b.data_ptr(), c.data_ptr(), per_sample, y1.data_ptr(),
-                                           y2.data_ptr(), g.data_ptr(), B, Nx, Ny, dt, h,
-                                           _lib.ptr(gb), _lib.ptr(gc), _lib.ptr(gy1),
-                                           _lib.ptr(gy2), _lib.stream_ptr(g.device)),
-                   "wt_step_bwd")
+        name = "wt_step_bwd_f64" if y1.dtype == torch.float64 else "wt_step_bwd"
+        _lib.check(getattr(_lib.lib(), name)(b.data_ptr(), c.data_ptr(), per_sample,
+                                             y1.data_ptr(), y2.data_ptr(), g.data_ptr(), B, Nx,
+                                             Ny, dt, h, _lib.ptr(gb), _lib.ptr(gc),
+                                             _lib.ptr(gy1), _lib.ptr(gy2),
+                                             _lib.stream_ptr(g.device)), name)
         # a shared (Nx,Ny) field broadcast against a per-sample one: reduce
         # over the batch like autograd's sum_to_size
         if gb is not None and gb.dim() > len(shapes[0]):

@@ -24,16 +24,15 @@ namespace wt {
 
 // 5-point Laplacian with zero padding, in conv2d's accumulation order
 // (operators.py:5-11): (((t1*up + t1*lf) + t4*ctr) + t1*rt) + t1*dn.
-template <bool UNIT_H>
-__device__ __forceinline__ float lap5(float up, float lf, float ct, float rt,
-                                      float dn, float t1, float t4) {
+template <bool UNIT_H, typename T>
+__device__ __forceinline__ T lap5(T up, T lf, T ct, T rt, T dn, T t1, T t4) {
   if (UNIT_H) {
-    float s = up + lf;
-    s = s + (-4.0f) * ct;
+    T s = up + lf;
+    s = s + T(-4) * ct;
     s = s + rt;
     return s + dn;
   } else {
-    float s = t1 * up;
+    T s = t1 * up;
     s = s + t1 * lf;
     s = s + t4 * ct;
     s = s + t1 * rt;
@@ -43,18 +42,19 @@ __device__ __forceinline__ float lap5(float up, float lf, float ct, float rt,
 
 // operators.py:5-11 alone: L u with zero padding, conv2d's order, for any h
 // (t1 = h^-2: with t1 = 1 each t1 * x is x exactly).
-__global__ void k_laplacian(const float *__restrict__ u, int B, int Nx, int Ny, float t1,
-                            float t4, float *__restrict__ out) {
+template <typename T>
+__global__ void k_laplacian(const T *__restrict__ u, int B, int Nx, int Ny, T t1, T t4,
+                            T *__restrict__ out) {
   const int64_t F = (int64_t)Nx * Ny;
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= F * B) return;
   const int64_t q = gid % F;
   const int i = (int)(q / Ny), j = (int)(q % Ny);
-  const float *v = u + (gid - q);
-  const float up = i > 0 ? v[q - Ny] : 0.0f;
-  const float dn = i < Nx - 1 ? v[q + Ny] : 0.0f;
-  const float lf = j > 0 ? v[q - 1] : 0.0f;
-  const float rt = j < Ny - 1 ? v[q + 1] : 0.0f;
+  const T *v = u + (gid - q);
+  const T up = i > 0 ? v[q - Ny] : T(0);
+  const T dn = i < Nx - 1 ? v[q + Ny] : T(0);
+  const T lf = j > 0 ? v[q - 1] : T(0);
+  const T rt = j < Ny - 1 ? v[q + 1] : T(0);
   out[gid] = lap5<false>(up, lf, v[q], rt, dn, t1, t4);
 }
 
@@ -73,48 +73,42 @@ __global__ void k_coef_linear(const float *__restrict__ b,
 
 // ------------------------------------------------------------- TimeStep
 
-template <bool UNIT_H>
-__global__ void k_step_fwd(const float *__restrict__ b,
-                           const float *__restrict__ c, int bc_ps,
-                           const float *__restrict__ y1,
-                           const float *__restrict__ y2, int B, int Nx, int Ny,
-                           float dtm1, float dtm2, float s2, float t1, float t4,
-                           float *__restrict__ y) {
+template <bool UNIT_H, typename T>
+__global__ void k_step_fwd(const T *__restrict__ b, const T *__restrict__ c, int bc_ps,
+                           const T *__restrict__ y1, const T *__restrict__ y2, int B, int Nx,
+                           int Ny, T dtm1, T dtm2, T s2, T t1, T t4, T *__restrict__ y) {
   int64_t F = (int64_t)Nx * Ny;
   int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= F * B) return;
   int64_t q = gid % F;
   int i = (int)(q / Ny), j = (int)(q % Ny);
-  const float *u = y1 + (gid - q);
-  float ct = u[q];
-  float up = i > 0 ? u[q - Ny] : 0.0f;
-  float dn = i < Nx - 1 ? u[q + Ny] : 0.0f;
-  float lf = j > 0 ? u[q - 1] : 0.0f;
-  float rt = j < Ny - 1 ? u[q + 1] : 0.0f;
-  float lap = lap5<UNIT_H>(up, lf, ct, rt, dn, t1, t4);
+  const T *u = y1 + (gid - q);
+  T ct = u[q];
+  T up = i > 0 ? u[q - Ny] : T(0);
+  T dn = i < Nx - 1 ? u[q + Ny] : T(0);
+  T lf = j > 0 ? u[q - 1] : T(0);
+  T rt = j < Ny - 1 ? u[q + 1] : T(0);
+  T lap = lap5<UNIT_H>(up, lf, ct, rt, dn, t1, t4);
   int64_t m = bc_ps ? gid : q;
-  float bd = b[m] * dtm1;
-  float A = 1.0f / (dtm2 + bd);
-  float Bm = dtm2 - bd;
-  float cc = c[m];
-  float c2 = cc * cc;
-  float p1 = s2 * ct;
-  float p2 = Bm * y2[gid];
-  float p3 = c2 * lap;
+  T bd = b[m] * dtm1;
+  T A = T(1) / (dtm2 + bd);
+  T Bm = dtm2 - bd;
+  T cc = c[m];
+  T c2 = cc * cc;
+  T p1 = s2 * ct;
+  T p2 = Bm * y2[gid];
+  T p3 = c2 * lap;
   y[gid] = A * ((p1 - p2) + p3);
 }
 
 // TimeStep.backward (cell.py:26-44).  grad_b / grad_c of shared (Nx,Ny)
 // material fields are summed over the batch in-thread (deterministic).
-template <bool UNIT_H>
-__global__ void k_step_bwd(const float *__restrict__ b,
-                           const float *__restrict__ c, int bc_ps,
-                           const float *__restrict__ y1,
-                           const float *__restrict__ y2,
-                           const float *__restrict__ g, int B, int Nx, int Ny,
-                           float dt, float t1, float t4, float *__restrict__ gb,
-                           float *__restrict__ gc, float *__restrict__ gy1,
-                           float *__restrict__ gy2) {
+template <bool UNIT_H, typename T>
+__global__ void k_step_bwd(const T *__restrict__ b, const T *__restrict__ c, int bc_ps,
+                           const T *__restrict__ y1, const T *__restrict__ y2,
+                           const T *__restrict__ g, int B, int Nx, int Ny, T dt, T t1, T t4,
+                           T *__restrict__ gb, T *__restrict__ gc, T *__restrict__ gy1,
+                           T *__restrict__ gy2) {
   int64_t F = (int64_t)Nx * Ny;
   int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // shared material: one thread per cell loops over the batch
@@ -123,46 +117,46 @@ __global__ void k_step_bwd(const float *__restrict__ b,
   int nb = bc_ps ? 1 : B;
   int64_t q = gid % F;
   int i = (int)(q / Ny), j = (int)(q % Ny);
-  float dt2 = dt * dt;
-  float accb = 0.0f, accc = 0.0f;
+  T dt2 = dt * dt;
+  T accb = T(0), accc = T(0);
   for (int k = 0; k < nb; ++k) {
     int64_t e = bc_ps ? gid : (int64_t)k * F + q;  // field element
     int64_t base = e - q;
     int64_t mq = bc_ps ? e : q;                     // material element
-    float bb = b[mq], cc = c[mq], gg = g[e];
-    float inv = 1.0f / (bb * dt + 1.0f);
+    T bb = b[mq], cc = c[mq], gg = g[e];
+    T inv = T(1) / (bb * dt + T(1));
     if (gb || gc) {
-      const float *u = y1 + base;
-      float up = i > 0 ? u[q - Ny] : 0.0f;
-      float dn = i < Nx - 1 ? u[q + Ny] : 0.0f;
-      float lf = j > 0 ? u[q - 1] : 0.0f;
-      float rt = j < Ny - 1 ? u[q + 1] : 0.0f;
-      float L = lap5<UNIT_H>(up, lf, u[q], rt, dn, t1, t4);
+      const T *u = y1 + base;
+      T up = i > 0 ? u[q - Ny] : T(0);
+      T dn = i < Nx - 1 ? u[q + Ny] : T(0);
+      T lf = j > 0 ? u[q - 1] : T(0);
+      T rt = j < Ny - 1 ? u[q + 1] : T(0);
+      T L = lap5<UNIT_H>(up, lf, u[q], rt, dn, t1, t4);
       if (gb) {
-        float t = cc * cc * dt2 * L + 2.0f * u[q] - 2.0f * y2[e];
-        float v = -(inv * inv) * dt * t * gg;
+        T t = cc * cc * dt2 * L + T(2) * u[q] - T(2) * y2[e];
+        T v = -(inv * inv) * dt * t * gg;
         if (bc_ps) gb[e] = v; else accb += v;
       }
       if (gc) {
-        float v = inv * (2.0f * cc * dt2 * L) * gg;
+        T v = inv * (T(2) * cc * dt2 * L) * gg;
         if (bc_ps) gc[e] = v; else accc += v;
       }
     }
     if (gy1) {
       // dt^2 * L(c^2 g / (1+b dt)) + 2 g / (1+b dt)
-      auto cg = [&](int64_t qq) -> float {
+      auto cg = [&](int64_t qq) -> T {
         int64_t mm = bc_ps ? base + qq : qq;
-        float b2 = b[mm], c2 = c[mm];
-        return (1.0f / (b2 * dt + 1.0f)) * (c2 * c2) * g[base + qq];
+        T b2 = b[mm], c2 = c[mm];
+        return (T(1) / (b2 * dt + T(1))) * (c2 * c2) * g[base + qq];
       };
-      float up = i > 0 ? cg(q - Ny) : 0.0f;
-      float dn = i < Nx - 1 ? cg(q + Ny) : 0.0f;
-      float lf = j > 0 ? cg(q - 1) : 0.0f;
-      float rt = j < Ny - 1 ? cg(q + 1) : 0.0f;
-      float L = lap5<UNIT_H>(up, lf, cg(q), rt, dn, t1, t4);
-      gy1[e] = dt2 * L + 2.0f * gg * inv;
+      T up = i > 0 ? cg(q - Ny) : T(0);
+      T dn = i < Nx - 1 ? cg(q + Ny) : T(0);
+      T lf = j > 0 ? cg(q - 1) : T(0);
+      T rt = j < Ny - 1 ? cg(q + 1) : T(0);
+      T L = lap5<UNIT_H>(up, lf, cg(q), rt, dn, t1, t4);
+      gy1[e] = dt2 * L + T(2) * gg * inv;
     }
-    if (gy2) gy2[e] = (bb * dt - 1.0f) * inv * gg;
+    if (gy2) gy2[e] = (bb * dt - T(1)) * inv * gg;
   }
   if (!bc_ps) {
     if (gb) gb[q] = accb;
@@ -476,44 +470,80 @@ hipError_t launch_coef_linear(const float *b, const float *c, int64_t n, float d
   return hipGetLastError();
 }
 
-hipError_t launch_laplacian(const float *u, int B, int Nx, int Ny, float t1, float t4,
-                            float *out, hipStream_t s) {
+template <typename T>
+static hipError_t laplacian_t(const T *u, int B, int Nx, int Ny, T t1, T t4, T *out,
+                              hipStream_t s) {
   int64_t n = (int64_t)B * Nx * Ny;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_laplacian, dim3(nblk(n, 256)), dim3(256), 0, s, u, B, Nx, Ny, t1, t4,
+  hipLaunchKernelGGL(k_laplacian<T>, dim3(nblk(n, 256)), dim3(256), 0, s, u, B, Nx, Ny, t1, t4,
                      out);
   return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t step_fwd_t(bool unit_h, const T *b, const T *c, int bc_ps, const T *y1,
+                             const T *y2, int B, int Nx, int Ny, T dtm1, T dtm2, T s2, T t1,
+                             T t4, T *y, hipStream_t s) {
+  int64_t n = (int64_t)B * Nx * Ny;
+  if (n <= 0) return hipSuccess;
+  if (unit_h)
+    hipLaunchKernelGGL((k_step_fwd<true, T>), dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps,
+                       y1, y2, B, Nx, Ny, dtm1, dtm2, s2, t1, t4, y);
+  else
+    hipLaunchKernelGGL((k_step_fwd<false, T>), dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps,
+                       y1, y2, B, Nx, Ny, dtm1, dtm2, s2, t1, t4, y);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t step_bwd_t(bool unit_h, const T *b, const T *c, int bc_ps, const T *y1,
+                             const T *y2, const T *g, int B, int Nx, int Ny, T dt, T t1, T t4,
+                             T *gb, T *gc, T *gy1, T *gy2, hipStream_t s) {
+  int64_t n = bc_ps ? (int64_t)B * Nx * Ny : (int64_t)Nx * Ny;
+  if (n <= 0) return hipSuccess;
+  // with shared material one thread walks the batch of a cell; the
+  // per-sample outputs grad_y1/grad_y2 are written inside that walk.
+  if (unit_h)
+    hipLaunchKernelGGL((k_step_bwd<true, T>), dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps,
+                       y1, y2, g, B, Nx, Ny, dt, t1, t4, gb, gc, gy1, gy2);
+  else
+    hipLaunchKernelGGL((k_step_bwd<false, T>), dim3(nblk(n, 256)), dim3(256), 0, s, b, c,
+                       bc_ps, y1, y2, g, B, Nx, Ny, dt, t1, t4, gb, gc, gy1, gy2);
+  return hipGetLastError();
+}
+
+hipError_t launch_laplacian(const float *u, int B, int Nx, int Ny, float t1, float t4,
+                            float *out, hipStream_t s) {
+  return laplacian_t(u, B, Nx, Ny, t1, t4, out, s);
+}
+hipError_t launch_laplacian(const double *u, int B, int Nx, int Ny, double t1, double t4,
+                            double *out, hipStream_t s) {
+  return laplacian_t(u, B, Nx, Ny, t1, t4, out, s);
 }
 
 hipError_t launch_step_fwd(bool unit_h, const float *b, const float *c, int bc_ps,
                            const float *y1, const float *y2, int B, int Nx, int Ny, float dtm1,
                            float dtm2, float s2, float t1, float t4, float *y, hipStream_t s) {
-  int64_t n = (int64_t)B * Nx * Ny;
-  if (n <= 0) return hipSuccess;
-  if (unit_h)
-    hipLaunchKernelGGL(k_step_fwd<true>, dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps, y1,
-                       y2, B, Nx, Ny, dtm1, dtm2, s2, t1, t4, y);
-  else
-    hipLaunchKernelGGL(k_step_fwd<false>, dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps, y1,
-                       y2, B, Nx, Ny, dtm1, dtm2, s2, t1, t4, y);
-  return hipGetLastError();
+  return step_fwd_t(unit_h, b, c, bc_ps, y1, y2, B, Nx, Ny, dtm1, dtm2, s2, t1, t4, y, s);
+}
+hipError_t launch_step_fwd(bool unit_h, const double *b, const double *c, int bc_ps,
+                           const double *y1, const double *y2, int B, int Nx, int Ny,
+                           double dtm1, double dtm2, double s2, double t1, double t4, double *y,
+                           hipStream_t s) {
+  return step_fwd_t(unit_h, b, c, bc_ps, y1, y2, B, Nx, Ny, dtm1, dtm2, s2, t1, t4, y, s);
 }
 
 hipError_t launch_step_bwd(bool unit_h, const float *b, const float *c, int bc_ps,
                            const float *y1, const float *y2, const float *g, int B, int Nx,
                            int Ny, float dt, float t1, float t4, float *gb, float *gc,
                            float *gy1, float *gy2, hipStream_t s) {
-  int64_t n = bc_ps ? (int64_t)B * Nx * Ny : (int64_t)Nx * Ny;
-  if (n <= 0) return hipSuccess;
-  // with shared material one thread walks the batch of a cell; the
-  // per-sample outputs grad_y1/grad_y2 are written inside that walk.
-  if (unit_h)
-    hipLaunchKernelGGL(k_step_bwd<true>, dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps, y1,
-                       y2, g, B, Nx, Ny, dt, t1, t4, gb, gc, gy1, gy2);
-  else
-    hipLaunchKernelGGL(k_step_bwd<false>, dim3(nblk(n, 256)), dim3(256), 0, s, b, c, bc_ps, y1,
-                       y2, g, B, Nx, Ny, dt, t1, t4, gb, gc, gy1, gy2);
-  return hipGetLastError();
+  return step_bwd_t(unit_h, b, c, bc_ps, y1, y2, g, B, Nx, Ny, dt, t1, t4, gb, gc, gy1, gy2, s);
+}
+hipError_t launch_step_bwd(bool unit_h, const double *b, const double *c, int bc_ps,
+                           const double *y1, const double *y2, const double *g, int B, int Nx,
+                           int Ny, double dt, double t1, double t4, double *gb, double *gc,
+                           double *gy1, double *gy2, hipStream_t s) {
+  return step_bwd_t(unit_h, b, c, bc_ps, y1, y2, g, B, Nx, Ny, dt, t1, t4, gb, gc, gy1, gy2, s);
 }
 
 hipError_t launch_fwd_plain(bool unit_h, int nl, bool rec, const FwdArgs &a,

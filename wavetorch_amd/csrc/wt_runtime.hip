@@ -42,6 +42,16 @@ hipError_t launch_coef_linear(const float *b, const float *c, int64_t n, float d
                               float *A, float *Bm, float *C2, hipStream_t s);
 hipError_t launch_laplacian(const float *u, int B, int Nx, int Ny, float t1, float t4,
                             float *out, hipStream_t s);
+hipError_t launch_laplacian(const double *u, int B, int Nx, int Ny, double t1, double t4,
+                            double *out, hipStream_t s);
+hipError_t launch_step_fwd(bool unit_h, const double *b, const double *c, int bc_ps,
+                           const double *y1, const double *y2, int B, int Nx, int Ny,
+                           double dtm1, double dtm2, double s2, double t1, double t4, double *y,
+                           hipStream_t s);
+hipError_t launch_step_bwd(bool unit_h, const double *b, const double *c, int bc_ps,
+                           const double *y1, const double *y2, const double *g, int B, int Nx,
+                           int Ny, double dt, double t1, double t4, double *gb, double *gc,
+                           double *gy1, double *gy2, hipStream_t s);
 hipError_t launch_step_fwd(bool unit_h, const float *b, const float *c, int bc_ps,
                            const float *y1, const float *y2, int B, int Nx, int Ny, float dtm1,
                            float dtm2, float s2, float t1, float t4, float *y, hipStream_t s);
@@ -415,16 +425,19 @@ int wt_last_error(char *buf, size_t len) {
   return (int)s.size();
 }
 
-static void scalars(float dt, float h, float &dtm1, float &dtm2, float &s2, float &t1,
-                    float &t4) {
-  // dt**-2 == 1/(dt*dt), dt**-1 == 1/dt, 2/dt**2 == 2*dt**-2 for the fp32
-  // 0-dim tensors of cell.py:13-14; h**(-2) == 1/(h*h) (operators.py:7).
-  dtm2 = 1.0f / (dt * dt);
-  dtm1 = 1.0f / dt;
-  s2 = 2.0f * dtm2;
-  t1 = 1.0f / (h * h);
-  t4 = -4.0f * t1;
+extern "C++" {
+template <typename T>
+static void scalars(T dt, T h, T &dtm1, T &dtm2, T &s2, T &t1, T &t4) {
+  // dt**-2 == 1/(dt*dt), dt**-1 == 1/dt, 2/dt**2 == 2*dt**-2 for the 0-dim
+  // tensors of cell.py:13-14; h**(-2) == 1/(h*h) (operators.py:7); in the
+  // compute type.
+  dtm2 = T(1) / (dt * dt);
+  dtm1 = T(1) / dt;
+  s2 = T(2) * dtm2;
+  t1 = T(1) / (h * h);
+  t4 = T(-4) * t1;
 }
+}  // extern "C++"
 
 int wt_check_finite(const float *p, int64_t n, int32_t *flag, void *stream) {
   if (!p || !flag || n < 0) {
@@ -456,6 +469,47 @@ int wt_laplacian(const float *u, float *out, int32_t B, int32_t Nx, int32_t Ny, 
   float dtm1, dtm2, s2, t1, t4;
   scalars(1.0f, h, dtm1, dtm2, s2, t1, t4);
   WT_HIP(launch_laplacian(u, B, Nx, Ny, t1, t4, out, (hipStream_t)stream));
+  return WT_OK;
+}
+
+int wt_laplacian_f64(const double *u, double *out, int32_t B, int32_t Nx, int32_t Ny, double h,
+                     void *stream) {
+  if (!u || !out || u == out || B < 0 || Nx <= 0 || Ny <= 0 || !(h > 0) || !std::isfinite(h)) {
+    set_error("wt_laplacian_f64: invalid argument");
+    return WT_EINVAL;
+  }
+  double dtm1, dtm2, s2, t1, t4;
+  scalars(1.0, h, dtm1, dtm2, s2, t1, t4);
+  WT_HIP(launch_laplacian(u, B, Nx, Ny, t1, t4, out, (hipStream_t)stream));
+  return WT_OK;
+}
+
+int wt_step_fwd_f64(const double *b, const double *c, int32_t bc_per_sample, const double *y1,
+                    const double *y2, int32_t B, int32_t Nx, int32_t Ny, double dt, double h,
+                    double *y, void *stream) {
+  if (!b || !c || !y1 || !y2 || !y || B < 0 || Nx <= 0 || Ny <= 0 || !(dt > 0) || !(h > 0)) {
+    set_error("wt_step_fwd_f64: invalid argument");
+    return WT_EINVAL;
+  }
+  double dtm1, dtm2, s2, t1, t4;
+  scalars(dt, h, dtm1, dtm2, s2, t1, t4);
+  WT_HIP(launch_step_fwd(h == 1.0, b, c, bc_per_sample, y1, y2, B, Nx, Ny, dtm1, dtm2, s2, t1,
+                         t4, y, (hipStream_t)stream));
+  return WT_OK;
+}
+
+int wt_step_bwd_f64(const double *b, const double *c, int32_t bc_per_sample, const double *y1,
+                    const double *y2, const double *g, int32_t B, int32_t Nx, int32_t Ny,
+                    double dt, double h, double *grad_b, double *grad_c, double *grad_y1,
+                    double *grad_y2, void *stream) {
+  if (!b || !c || !y1 || !y2 || !g || B < 0 || Nx <= 0 || Ny <= 0 || !(dt > 0) || !(h > 0)) {
+    set_error("wt_step_bwd_f64: invalid argument");
+    return WT_EINVAL;
+  }
+  double dtm1, dtm2, s2, t1, t4;
+  scalars(dt, h, dtm1, dtm2, s2, t1, t4);
+  WT_HIP(launch_step_bwd(h == 1.0, b, c, bc_per_sample, y1, y2, g, B, Nx, Ny, dt, t1, t4,
+                         grad_b, grad_c, grad_y1, grad_y2, (hipStream_t)stream));
   return WT_OK;
 }
 

@@ -128,31 +128,10 @@ class WaveGeometryHoley(WaveGeometry):
         return cached[1]
 
     def _holes(self, c0, c1):
-        if self.x.dtype == torch.float64:
-            _require_device(self.x, "WaveGeometryHoley")
-            return self._holes_float64(c0, c1)
         _require_hip(self.x, "WaveGeometryHoley")
         eta, beta, _, _ = self._host_params()
         Nx, Ny = self.domain_shape
         return _HoleyC.apply(self.x, self.y, self.r, Nx, Ny, eta, beta, c0, c1)
-
-    def _holes_float64(self, c0, c1):
-        """geom.py:109-135 in float64 on the device: the hole sum with the
-        reference's torch ops, projection, c0 + (c1 - c0) * rho (c0 = 0,
-        c1 = 1 gives rho itself)."""
-        eta, beta, _, _ = self._host_params()
-        Nx, Ny = self.domain_shape
-        dev, dt = self.x.device, self.x.dtype
-        xg, yg = torch.meshgrid(torch.arange(0, Nx, dtype=dt, device=dev),
-                                torch.arange(0, Ny, dtype=dt, device=dev), indexing="ij")
-        acc = torch.zeros((Nx, Ny), dtype=dt, device=dev)
-        for ri, xi, yi in zip(self.r, self.x, self.y):
-            d = torch.sqrt((xg - xi).pow(2) + (yg - yi).pow(2))
-            acc = acc + torch.exp(-d / ri)
-        rho = _projection(acc, eta, beta)
-        if c0 == 0.0 and c1 == 1.0:
-            return rho
-        return c0 + (c1 - c0) * rho
 
     def _rho(self):
         """Projected density P(sum_h exp(-d_h / r_h)) (geom.py:109-128), gfx950
@@ -183,11 +162,14 @@ class _HoleyC(torch.autograd.Function):
             raise ValueError("WaveGeometryHoley: x, y and r must have one entry per hole")
         for t in (hy, hr):
             _require_hip(t, "WaveGeometryHoley")
-        c = torch.empty((Nx, Ny), dtype=torch.float32, device=hx.device)
+            if t.dtype != hx.dtype:
+                raise TypeError("WaveGeometryHoley: x, y and r must share one dtype")
+        c = torch.empty((Nx, Ny), dtype=hx.dtype, device=hx.device)
         raw = torch.empty_like(c)
-        _lib.check(_lib.lib().wt_geom_holey_c(
+        name = "wt_geom_holey_c_f64" if hx.dtype == torch.float64 else "wt_geom_holey_c"
+        _lib.check(getattr(_lib.lib(), name)(
             hx.data_ptr(), hy.data_ptr(), hr.data_ptr(), H, Nx, Ny, eta, beta, c0, c1,
-            c.data_ptr(), raw.data_ptr(), _lib.stream_ptr(hx.device)), "wt_geom_holey_c")
+            c.data_ptr(), raw.data_ptr(), _lib.stream_ptr(hx.device)), name)
         ctx.save_for_backward(raw, hx, hy, hr)
         ctx.params = (Nx, Ny, eta, beta, c0, c1)
         ctx.shapes = shapes
@@ -202,14 +184,16 @@ class _HoleyC(torch.autograd.Function):
         need = ctx.needs_input_grad[:3]
         out = [torch.empty_like(t) if n else None for t, n in zip((hx, hy, hr), need)]
         L = _lib.lib()
+        suffix = "_f64" if raw.dtype == torch.float64 else ""
         nbytes = ctypes.c_int64()
-        _lib.check(L.wt_geom_holey_workspace_bytes(H, Nx, Ny, ctypes.byref(nbytes)),
-                   "wt_geom_holey_workspace_bytes")
+        _lib.check(getattr(L, "wt_geom_holey_workspace_bytes" + suffix)(
+            H, Nx, Ny, ctypes.byref(nbytes)), "wt_geom_holey_workspace_bytes" + suffix)
         work = torch.empty((nbytes.value + 7) // 8, dtype=torch.float64, device=raw.device)
-        _lib.check(L.wt_geom_holey_c_bwd(
-            grad_c.contiguous().data_ptr(), raw.data_ptr(), hx.data_ptr(), hy.data_ptr(),
-            hr.data_ptr(), H, Nx, Ny, eta, beta, c0, c1, *(_lib.ptr(o) for o in out),
-            work.data_ptr(), _lib.stream_ptr(raw.device)), "wt_geom_holey_c_bwd")
+        _lib.check(getattr(L, "wt_geom_holey_c_bwd" + suffix)(
+            grad_c.to(raw.dtype).contiguous().data_ptr(), raw.data_ptr(), hx.data_ptr(),
+            hy.data_ptr(), hr.data_ptr(), H, Nx, Ny, eta, beta, c0, c1,
+            *(_lib.ptr(o) for o in out), work.data_ptr(), _lib.stream_ptr(raw.device)),
+            "wt_geom_holey_c_bwd" + suffix)
         grads = [o.reshape(s) if o is not None else None for o, s in zip(out, ctx.shapes)]
         return (*grads, None, None, None, None, None, None)
 
@@ -235,13 +219,16 @@ class _FreeFormC(torch.autograd.Function):
         rho = rho.contiguous()
         kernel = kernel.contiguous()
         Nx, Ny = rho.shape
+        if kernel.dtype != rho.dtype:
+            raise TypeError("WaveGeometryFreeForm: blur kernel and rho must share one dtype")
         c = torch.empty_like(rho)
         blur = torch.empty_like(rho)
         work = torch.empty(2 * Nx * Ny, dtype=rho.dtype, device=rho.device)
         s = _lib.stream_ptr(rho.device)
-        _lib.check(_lib.lib().wt_geom_freeform_c(
+        name = "wt_geom_freeform_c_f64" if rho.dtype == torch.float64 else "wt_geom_freeform_c"
+        _lib.check(getattr(_lib.lib(), name)(
             rho.data_ptr(), kernel.data_ptr(), radius, blur_N, Nx, Ny, eta, beta, c0, c1,
-            c.data_ptr(), None, blur.data_ptr(), work.data_ptr(), s), "wt_geom_freeform_c")
+            c.data_ptr(), None, blur.data_ptr(), work.data_ptr(), s), name)
         ctx.save_for_backward(blur, kernel)
         ctx.params = (radius, blur_N, eta, beta, c0, c1)
         return c
@@ -250,35 +237,24 @@ class _FreeFormC(torch.autograd.Function):
     def backward(ctx, grad_c):
         blur, kernel = ctx.saved_tensors
         radius, blur_N, eta, beta, c0, c1 = ctx.params
-        grad_c = grad_c.contiguous()
+        grad_c = grad_c.to(blur.dtype).contiguous()
         Nx, Ny = blur.shape
         g = torch.empty_like(blur)
         work = torch.empty(2 * Nx * Ny, dtype=blur.dtype, device=blur.device)
         s = _lib.stream_ptr(blur.device)
-        _lib.check(_lib.lib().wt_geom_freeform_c_bwd(
+        name = ("wt_geom_freeform_c_bwd_f64" if blur.dtype == torch.float64
+                else "wt_geom_freeform_c_bwd")
+        _lib.check(getattr(_lib.lib(), name)(
             grad_c.data_ptr(), blur.data_ptr(), kernel.data_ptr(), radius, blur_N, Nx, Ny, eta,
-            beta, c0, c1, g.data_ptr(), work.data_ptr(), s), "wt_geom_freeform_c_bwd")
+            beta, c0, c1, g.data_ptr(), work.data_ptr(), s), name)
         return g, None, None, None, None, None, None, None
-
-
-def _require_device(t, what):
-    if not t.is_cuda:
-        raise RuntimeError(
-            f"wavetorch_amd: {what} runs on the HIP device only; move the model with .to('cuda')")
-
-
-def _projection(x, eta, beta):
-    """The tanh projection of geom.py:217-222 with host-side tanh(beta*eta)
-    and tanh(beta*(1-eta)) (numpy, as the reference evaluates them)."""
-    te = np.tanh(beta * eta)
-    return (te + torch.tanh(beta * (x - eta))) / (te + np.tanh(beta * (1 - eta)))
 
 
 def _require_hip(t, what):
     if not t.is_cuda:
         raise RuntimeError(
             f"wavetorch_amd: {what} runs on the HIP device only; move the model with .to('cuda')")
-    if t.dtype != torch.float32:
+    if t.dtype not in (torch.float32, torch.float64):
         raise NotImplementedError(f"wavetorch_amd: {what} computes in float32 or float64 "
                                   f"(got {t.dtype})")
 
@@ -378,23 +354,9 @@ class WaveGeometryFreeForm(WaveGeometry):
 
     @property
     def c(self):
-        """c0 + (c1-c0) * P(Blur^N(rho)) (geom.py:231-233), gfx950 kernels
-        (float32); in float64 the reference's torch ops on the device."""
-        if self.rho.dtype == torch.float64:
-            _require_device(self.rho, "WaveGeometryFreeForm.c")
-            return self._c_float64()
+        """c0 + (c1-c0) * P(Blur^N(rho)) (geom.py:231-233), gfx950 kernels in
+        the geometry's dtype (float32, or float64 after set_dtype)."""
         _require_hip(self.rho, "WaveGeometryFreeForm.c")
-        k = self.blur_kernel[0, 0].to(device=self.rho.device, dtype=torch.float32)
+        k = self.blur_kernel[0, 0].to(device=self.rho.device, dtype=self.rho.dtype)
         blur_N, eta, beta, c0, c1 = self._host_params()
         return _FreeFormC.apply(self.rho, k, self._blur_radius, blur_N, eta, beta, c0, c1)
-
-    def _c_float64(self):
-        """geom.py:207-233 in float64 on the device: blur_N passes of conv2d
-        with the disk kernel (zero padding), projection, c0 + (c1 - c0) P."""
-        blur_N, eta, beta, c0, c1 = self._host_params()
-        rho = self.rho
-        k = self.blur_kernel.to(device=rho.device, dtype=rho.dtype)
-        for _ in range(blur_N):
-            rho = torch.nn.functional.conv2d(rho.unsqueeze(0).unsqueeze(0), k,
-                                             padding=k.shape[-1] // 2).squeeze()
-        return c0 + (c1 - c0) * _projection(rho, eta, beta)
