@@ -60,6 +60,9 @@ int wt_last_error(char *buf, size_t len);
  * wavetorch_amd runs it on the loop's outputs and gradients when a model's
  * check_finite is set. */
 int wt_check_finite(const float *p, int64_t n, int32_t *flag, void *stream);
+/* wt_check_finite over n float64 values (the float64 loop's outputs and
+ * gradients). */
+int wt_check_finite_f64(const double *p, int64_t n, int32_t *flag, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Material coefficients                                                     */
@@ -272,7 +275,8 @@ int wt_workspace_bytes_f64(const void *plan, int32_t with_grad, int64_t *saved,
 /* wt_loop_fwd in float64 (rnn.py:21-72 under set_dtype('float64')): one
  * launch per step, the reference's operation order in double precision.
  * Arguments as wt_loop_fwd, all arrays double; `sc` gives dt, h and the
- * nonlinear coefficients in double. */
+ * nonlinear coefficients in double.  With with_grad the fields go to `saved`
+ * and `temp` may be NULL. */
 int wt_loop_fwd_f64(void *plan, const wt_scalars_f64 *sc, const double *b, const double *c,
                     const double *rho, const double *x, double *y, double *fields,
                     int32_t field_stride, int32_t with_grad, void *saved, void *temp,

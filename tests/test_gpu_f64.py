@@ -205,3 +205,32 @@ def test_holey_geometry_float64(cuda):
             assert rel(p.grad.cpu().numpy(), q.grad.numpy()) < 1e-10
     finally:
         set_dtype("float32")
+
+
+def test_nonfinite_flags_float64(cuda):
+    """check_finite in float64 (wt_check_finite_f64): flags stay 0 for a
+    finite run; a NaN input sample sets the output flag, an infinite dJ/dy
+    the gradient flag."""
+    set_dtype("float64")
+    try:
+        from wavetorch_amd import workloads
+        from wavetorch_amd.utils import normalize_power
+        model, x, labels = workloads.propagate_100_b4(T=48)
+        model = model.to(cuda)
+        model.check_finite = True
+        x, labels = x.double().to(cuda), labels.to(cuda)
+        y = model(x)
+        torch.nn.functional.cross_entropy(normalize_power(y.sum(dim=1)), labels).backward()
+        assert model.nonfinite.tolist() == [0, 0]
+        xb = x.clone()
+        xb[1, 10] = float("nan")
+        model(xb)
+        assert model.nonfinite.tolist() == [1, 0]
+        model.nonfinite.zero_()
+        y = model(x)
+        g = torch.zeros_like(y)
+        g[0, 5, 1] = float("inf")
+        y.backward(g)
+        assert model.nonfinite.tolist() == [0, 1]
+    finally:
+        set_dtype("float32")

@@ -54,7 +54,8 @@ def test_entry_points_reject_null_arguments():
     message naming the entry point (include/wavetorch_hip.h status codes)."""
     import ctypes
     L = _lib.lib()
-    names = ["wt_laplacian", "wt_coef_linear", "wt_step_fwd", "wt_step_bwd", "wt_plan_resolve",
+    names = ["wt_laplacian", "wt_check_finite", "wt_check_finite_f64", "wt_coef_linear",
+             "wt_step_fwd", "wt_step_bwd", "wt_plan_resolve",
              "wt_plan_create", "wt_plan_tuning", "wt_workspace_bytes", "wt_loop_fwd",
              "wt_loop_adj", "wt_geom_freeform_c", "wt_geom_freeform_c_bwd",
              "wt_geom_holey_c", "wt_geom_holey_c_bwd", "wt_geom_holey_workspace_bytes",

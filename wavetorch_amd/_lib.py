@@ -71,6 +71,7 @@ _SIGS = {
     "wt_abi_version": (ctypes.c_int, []),
     "wt_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     "wt_check_finite": (ctypes.c_int, [c_p, c_i64, c_p, c_p]),
+    "wt_check_finite_f64": (ctypes.c_int, [c_p, c_i64, c_p, c_p]),
     "wt_coef_linear": (ctypes.c_int, [c_p, c_p, c_i64, c_f32, c_p, c_p, c_p, c_p]),
     "wt_laplacian": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_f32, c_p]),
     "wt_laplacian_f64": (ctypes.c_int, [c_p, c_p, c_i32, c_i32, c_i32, c_f64, c_p]),

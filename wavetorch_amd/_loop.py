@@ -128,11 +128,9 @@ def _buf(nbytes, device):
 
 def _check_finite(t, flag, slot, stream):
     if t is not None and flag is not None:
-        if t.dtype == torch.float64:  # on the device as well, no host synchronisation
-            flag[slot] = torch.maximum(flag[slot], (~torch.isfinite(t)).any().to(flag.dtype))
-            return
-        _lib.check(_lib.lib().wt_check_finite(t.data_ptr(), t.numel(), flag.data_ptr() + 4 * slot,
-                                              stream), "wt_check_finite")
+        name = "wt_check_finite_f64" if t.dtype == torch.float64 else "wt_check_finite"
+        _lib.check(getattr(_lib.lib(), name)(t.data_ptr(), t.numel(),
+                                             flag.data_ptr() + 4 * slot, stream), name)
 
 
 class WaveLoop(torch.autograd.Function):
@@ -173,12 +171,14 @@ class WaveLoop(torch.autograd.Function):
         if f64:
             sb, tb = plan.bytes_f64(with_grad)
             saved = _buf(sb, dev) if with_grad else None
-            temp = _buf(plan.bytes_f64(False)[1], dev)
+            # three rotating fields without gradients; the tape holds every
+            # field otherwise
+            temp = None if with_grad else _buf(plan.bytes_f64(False)[1], dev)
             sc = scalars_f64(plan.cfg)
             _lib.check(_lib.lib().wt_loop_fwd_f64(
                 plan.handle, ctypes.byref(sc), b.data_ptr(), c.data_ptr(), _lib.ptr(rho_c),
                 x.data_ptr(), _lib.ptr(y), _lib.ptr(fields), int(field_stride), int(with_grad),
-                _lib.ptr(saved), temp.data_ptr(), s), "wt_loop_fwd_f64")
+                _lib.ptr(saved), _lib.ptr(temp), s), "wt_loop_fwd_f64")
         else:
             saved = _buf(plan.saved_bytes, dev) if with_grad else None
             temp = _buf(plan.temp_bytes_nograd, dev)  # the forward's scratch, with or without grad

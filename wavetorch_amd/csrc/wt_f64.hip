@@ -349,7 +349,7 @@ extern "C" int wt_loop_fwd_f64(void *plan, const wt_scalars_f64 *sc, const doubl
                                double *fields, int32_t field_stride, int32_t with_grad,
                                void *saved, void *temp, void *stream) {
   PlanView v;
-  if (!plan || !sc || !b || !c || !x || !temp || (with_grad && !saved) ||
+  if (!plan || !sc || !b || !c || !x || (with_grad ? !saved : !temp) ||
       plan_view(plan, &v) != WT_OK || ((v.nlf & 3) && !rho)) {
     set_error("wt_loop_fwd_f64: invalid argument");
     return WT_EINVAL;

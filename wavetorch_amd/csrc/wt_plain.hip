@@ -450,11 +450,21 @@ __global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
 // flag[0] = 1 when any of the n values is NaN or infinite (exponent all
 // ones); left unchanged otherwise.  Every writer stores the same value, so no
 // atomic is needed.
-__global__ void k_nonfinite(const float *__restrict__ p, int64_t n, int32_t *__restrict__ flag) {
+// NaN or infinity: every exponent bit set
+__device__ __forceinline__ bool nonfinite_bits(float v) {
+  return (__float_as_uint(v) & 0x7f800000u) == 0x7f800000u;
+}
+__device__ __forceinline__ bool nonfinite_bits(double v) {
+  return ((unsigned long long)__double_as_longlong(v) & 0x7ff0000000000000ull) ==
+         0x7ff0000000000000ull;
+}
+
+template <typename T>
+__global__ void k_nonfinite(const T *__restrict__ p, int64_t n, int32_t *__restrict__ flag) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   bool bad = false;
-  for (; i < n; i += stride) bad |= (__float_as_uint(p[i]) & 0x7f800000u) == 0x7f800000u;
+  for (; i < n; i += stride) bad |= nonfinite_bits(p[i]);
   if (bad) flag[0] = 1;
 }
 
@@ -641,12 +651,19 @@ hipError_t launch_finalize_gx(const float *gsrc, int B, int T, int nslot, const 
   return hipGetLastError();
 }
 
-hipError_t launch_nonfinite(const float *p, int64_t n, int32_t *flag, hipStream_t s) {
+template <typename T>
+static hipError_t nonfinite_t(const T *p, int64_t n, int32_t *flag, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int64_t blocks = nblk(n, 256);
   if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)blocks), dim3(256), 0, s, p, n, flag);
+  hipLaunchKernelGGL(k_nonfinite<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, n, flag);
   return hipGetLastError();
+}
+hipError_t launch_nonfinite(const float *p, int64_t n, int32_t *flag, hipStream_t s) {
+  return nonfinite_t(p, n, flag, s);
+}
+hipError_t launch_nonfinite(const double *p, int64_t n, int32_t *flag, hipStream_t s) {
+  return nonfinite_t(p, n, flag, s);
 }
 
 hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t s) {

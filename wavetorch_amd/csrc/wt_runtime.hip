@@ -80,6 +80,7 @@ hipError_t launch_finalize_gx(const float *gsrc, int B, int T, int nslot, const 
                               float *gx, hipStream_t s);
 hipError_t launch_fill(float *p, int64_t n, float v, hipStream_t s);
 hipError_t launch_nonfinite(const float *p, int64_t n, int32_t *flag, hipStream_t s);
+hipError_t launch_nonfinite(const double *p, int64_t n, int32_t *flag, hipStream_t s);
 hipError_t launch_fwd_tiled(int nt, int cpt, bool unit_h, int nl, bool rec, const FwdArgs &a,
                             int LWmax, hipStream_t s);
 hipError_t launch_adj_tiled(int nt, int cpt, bool unit_h, const AdjArgs &a, int LWmax,
@@ -442,6 +443,15 @@ static void scalars(T dt, T h, T &dtm1, T &dtm2, T &s2, T &t1, T &t4) {
 int wt_check_finite(const float *p, int64_t n, int32_t *flag, void *stream) {
   if (!p || !flag || n < 0) {
     set_error("wt_check_finite: invalid argument");
+    return WT_EINVAL;
+  }
+  WT_HIP(launch_nonfinite(p, n, flag, (hipStream_t)stream));
+  return WT_OK;
+}
+
+int wt_check_finite_f64(const double *p, int64_t n, int32_t *flag, void *stream) {
+  if (!p || !flag || n < 0) {
+    set_error("wt_check_finite_f64: invalid argument");
     return WT_EINVAL;
   }
   WT_HIP(launch_nonfinite(p, n, flag, (hipStream_t)stream));
