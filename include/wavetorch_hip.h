@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define WT_ABI_VERSION 1
+#define WT_ABI_VERSION 2
 
 enum wt_status {
   WT_OK = 0,
@@ -163,6 +163,9 @@ typedef struct wt_tuning {
   int32_t tcols;     /* column tiles per sample (column-segment family; 0 = 1) */
   /* adjoint kernel choice; 0 = same as the forward fields above */
   int32_t a_strips, a_tcols, a_tblock, a_nt, a_cpt;
+  /* batched instances: the batch runs as this many concurrent parts on as
+     many streams (0 = library choice, at most 4); single plans ignore it */
+  int32_t streams;
 } wt_tuning;
 
 /* Creates a plan for `cfg` at batch B and length T: validates the
@@ -319,9 +322,17 @@ typedef struct wt_batch_group_io {
 /* ngroups configurations; B[g] instances in group g; lengths = the steps of
  * every instance, group after group (sum of B entries); cmax[g] for the CFL
  * check of group g.  tunings: NULL (library choice) or ngroups entries whose
- * tblock/nt/cpt/a_tblock/a_nt/a_cpt/ck_every are taken from entry 0 (common
- * to every launch) and strips/tcols (and a_strips/a_tcols) per group.  Only
- * the linear cell with h = 1 is batched. */
+ * tblock/nt/cpt/a_tblock/a_nt/a_cpt/ck_every/streams are taken from entry 0
+ * (common to every launch) and strips/tcols (and a_strips/a_tcols) per
+ * group.  Only the linear cell with h = 1 is batched.
+ * The plan may split the batch into `streams` parts (contiguous instance
+ * ranges of every group, balanced by steps; by default one part per ~768
+ * work items of a launch, at most 3): each part runs on a stream of the
+ * plan's own, forked from the caller's stream and joined back to it with
+ * events, so a captured graph holds all of them and results are those of
+ * the single sequence (the instances are independent).  Launches of the
+ * parts overlap, filling the CUs one part's launch leaves idle at its end.
+ * Calls with one plan must not overlap (the part streams are the plan's). */
 int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
                          const int32_t *lengths, const float *cmax,
                          const wt_tuning *tunings, void **plan_out);

@@ -98,3 +98,61 @@ def test_batch_graph_replay(cuda):
                 assert torch.equal(gc0, gc)
             results.append(eager[0][1].sum().item())
     assert len(set(results)) == 3  # the three inputs differ, so do the gradients
+
+
+@pytest.mark.parametrize("parts", [2, 3, 4])
+def test_batch_parts_match_single(parts, cuda):
+    """The batch split into concurrent parts on the plan's own streams gives
+    what the single launch sequence gives: probe series bit for bit, gradients
+    to the oracle's tolerance.  Four parts over groups of three instances
+    leave some parts without a group's instances."""
+    items = _instances(seed=2)
+    cs = [torch.from_numpy(it[1]).to(cuda) for it in items]
+    xs = [torch.from_numpy(it[2]).to(cuda) for it in items]
+    ybars = [-torch.ones((it[0].length, 1), device=cuda) for it in items]
+    runs = []
+    for st in (1, parts):
+        bt = InstanceBatch([it[0] for it in items], cuda, tuning={"streams": st})
+        print("case parts", st, bt.tuning[0], flush=True)
+        assert bt.tuning[0]["streams"] <= st
+        bt.set_inputs(cs, xs, ybars)
+        bt.fwd_adj()
+        torch.cuda.synchronize()
+        runs.append([(y.clone(), gc.clone(), gx.clone()) for (y, gc, gx) in bt.outputs()])
+        del bt
+    assert len(runs[0]) == len(runs[1]) == len(items)
+    for (inst, c, x), (y1, gc1, gx1), (yp, gcp, gxp) in zip(items, runs[0], runs[1]):
+        assert torch.equal(y1, yp), inst.length
+        _, gco, gxo = _oracle(inst, c, x, -np.ones((inst.length, 1)))
+        assert rel(gcp.cpu().numpy(), gco) < 1e-4, inst.length
+        assert rel(gxp.cpu().numpy(), gxo) < 1e-4, inst.length
+
+
+def test_batch_parts_graph_replay(cuda):
+    """A captured graph of a three-part batch (fork and join events, part
+    streams) replays the eager result, with eager work between replays."""
+    items = _instances(seed=3)
+    bt = InstanceBatch([it[0] for it in items], cuda, tuning={"streams": 3})
+    assert bt.tuning[0]["streams"] == 3
+    xs = [torch.from_numpy(it[2]).to(cuda) for it in items]
+    ybars = [-torch.ones((it[0].length, 1), device=cuda) for it in items]
+    s = torch.cuda.Stream(device=cuda)
+    with torch.cuda.stream(s):
+        for rep in range(3):
+            scale = 1.0 - 0.05 * rep
+            cs = [torch.from_numpy(it[1]).to(cuda) * scale for it in items]
+            bt.set_inputs(cs, xs, ybars)
+            bt.fwd_adj()
+            s.synchronize()
+            eager = [(y.clone(), gc.clone(), gx.clone()) for (y, gc, gx) in bt.outputs()]
+            if rep == 0:
+                bt.capture()
+            for g in bt.buf:
+                g["y"].fill_(float("nan"))
+                g["grad_c"].fill_(float("nan"))
+            bt.replay()
+            s.synchronize()
+            for (y0, gc0, gx0), (y, gc, gx) in zip(eager, bt.outputs()):
+                assert torch.equal(y0, y)
+                assert torch.equal(gc0, gc)
+                assert torch.equal(gx0, gx)

@@ -20,7 +20,7 @@ WT_ECFL = -2
 WT_EHIP = -3
 WT_EUNSUPPORTED = -4
 WT_ENOMEM = -5
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
@@ -45,10 +45,13 @@ class WtTuning(ctypes.Structure):
     _fields_ = [("plain", c_i32), ("strips", c_i32), ("tblock", c_i32), ("nt", c_i32),
                 ("cpt", c_i32), ("ck_every", c_i32), ("kind", c_i32),
                 ("tcols", c_i32), ("a_strips", c_i32), ("a_tcols", c_i32), ("a_tblock", c_i32),
-                ("a_nt", c_i32), ("a_cpt", c_i32)]
+                ("a_nt", c_i32), ("a_cpt", c_i32), ("streams", c_i32)]
 
     def as_dict(self):
-        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+        d = {k: int(getattr(self, k)) for k, _ in self._fields_}
+        if d["streams"] == 0:  # single plans: no part streams
+            del d["streams"]
+        return d
 
 
 class WtScalarsF64(ctypes.Structure):

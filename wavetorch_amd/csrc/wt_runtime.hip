@@ -1650,28 +1650,17 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
   return WT_OK;
 }
 
-bool batch_io_ok(const BatchPlan *bp, const wt_batch_group_io *io, bool adj) {
-  if (!bp || !io) return false;
-  for (int g = 0; g < bp->ngrp; ++g) {
-    if (!io[g].b || !io[g].c || !io[g].x) return false;
-    if (!adj && !io[g].y) return false;
-    if (adj && !io[g].ybar) return false;
-  }
-  return true;
-}
-
 }  // namespace
 
 extern "C" {
 
-int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
-                         const int32_t *lengths, const float *cmax, const wt_tuning *tunings,
-                         void **plan_out) {
-  if (!plan_out || !cfgs || !B || !lengths || !cmax || ngroups <= 0 ||
-      ngroups > WT_MAX_GROUPS) {
-    set_error("wt_batch_plan_create: invalid argument (1 <= ngroups <= %d)", WT_MAX_GROUPS);
-    return WT_EINVAL;
-  }
+// One batch plan over the instances given (the body of
+// wt_batch_plan_create); T_force > 0 sets the plan's length T (the row stride
+// of x, y, ybar and grad_x) to that of the whole batch when this plan is a
+// part of it.
+static int batch_build(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
+                       const int32_t *lengths, const float *cmax, const wt_tuning *tunings,
+                       int T_force, BatchPlan **plan_out) {
   *plan_out = nullptr;
   BatchPlan *bp = new BatchPlan();
   bp->ngrp = ngroups;
@@ -1707,6 +1696,7 @@ int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *
     bp->T = std::max(bp->T, tg);
     off += B[g];
   }
+  if (T_force > bp->T) bp->T = T_force;
   bp->K = std::min(bp->K, bp->T);
   bp->nseg = (bp->T + bp->K - 1) / bp->K;
   bp->tb = std::min(bp->tb, bp->K);
@@ -1781,63 +1771,295 @@ int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *
   return WT_OK;
 }
 
+
+// The handle of wt_batch_plan_create: the batch as `parts` concurrent batch
+// plans, each over a contiguous range of every group's instances (balanced by
+// steps), all with the batch's length T, so that a part's arrays are the
+// caller's group arrays at an offset.  With one part it runs on the caller's
+// stream; with several, each part runs on a stream of the handle's own,
+// forked from the caller's stream and joined back to it with events (a
+// captured graph holds both).  Kernel launches of the parts overlap: one
+// part's work items fill the CUs that another's launch leaves idle at its
+// end (a launch of ~2400 work items keeps ~81% of the CU time busy).
+struct BatchHandle {
+  int ngrp = 0, T = 0;
+  std::vector<BatchPlan *> parts;
+  std::vector<std::vector<int>> grp;  // per part: original group of each part group
+  std::vector<std::vector<int>> lo;   // per part: first instance in that group
+  std::vector<int> ntap;              // per original group
+  std::vector<int64_t> F;             // per original group: Nx * Ny
+  std::vector<size_t> soff, toff;
+  size_t saved_bytes = 0, temp_bytes = 0;
+  std::vector<hipStream_t> streams;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+};
+
+static void free_handle(BatchHandle *h) {
+  if (!h) return;
+  for (BatchPlan *bp : h->parts) free_batch(bp);
+  for (hipStream_t s : h->streams)
+    if (s) (void)hipStreamDestroy(s);
+  if (h->fork) (void)hipEventDestroy(h->fork);
+  for (hipEvent_t e : h->join)
+    if (e) (void)hipEventDestroy(e);
+  delete h;
+}
+
+// Library choice of the number of parts: one per ~768 work items of the
+// first launch (three rounds of 256 workgroups), at most 3 (measured on the
+// 256-instance lens batch, ~2400 work items: 1, 2, 3, 4 parts took 130.3,
+// 118.3, 113.1, 116.1 ms).
+static int default_parts(const BatchPlan &bp) {
+  int64_t items = 0;
+  for (const Plan *p : bp.g) items += (int64_t)p->B * p->ntiles;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(3, items / 768));
+}
+
+int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
+                         const int32_t *lengths, const float *cmax, const wt_tuning *tunings,
+                         void **plan_out) {
+  if (!plan_out || !cfgs || !B || !lengths || !cmax || ngroups <= 0 ||
+      ngroups > WT_MAX_GROUPS) {
+    set_error("wt_batch_plan_create: invalid argument (1 <= ngroups <= %d)", WT_MAX_GROUPS);
+    return WT_EINVAL;
+  }
+  *plan_out = nullptr;
+  BatchPlan *whole = nullptr;
+  int rc = batch_build(ngroups, cfgs, B, lengths, cmax, tunings, 0, &whole);
+  if (rc) return rc;
+  BatchHandle *h = new BatchHandle();
+  h->ngrp = ngroups;
+  h->T = whole->T;
+  for (const Plan *p : whole->g) {
+    h->ntap.push_back(p->ntap);
+    h->F.push_back((int64_t)p->F);
+  }
+  const int nparts = (tunings && tunings[0].streams > 0) ? std::min(tunings[0].streams, 4)
+                                                         : default_parts(*whole);
+  if (nparts <= 1) {
+    h->parts.push_back(whole);
+    h->grp.emplace_back();
+    h->lo.emplace_back();
+    for (int g = 0; g < ngroups; ++g) {
+      h->grp[0].push_back(g);
+      h->lo[0].push_back(0);
+    }
+  } else {
+    free_batch(whole);
+    // group g's instances [cut[g][k], cut[g][k+1]) go to part k, cut at the
+    // instance boundary nearest to k/nparts of the group's total steps
+    std::vector<std::vector<int>> cut(ngroups);
+    std::vector<int> goff(ngroups, 0);
+    for (int g = 0; g < ngroups; ++g) {
+      if (g > 0) goff[g] = goff[g - 1] + B[g - 1];
+      int64_t tot = 0;
+      for (int i = 0; i < B[g]; ++i) tot += lengths[goff[g] + i];
+      cut[g].push_back(0);
+      int64_t run = 0;
+      for (int i = 0, k = 1; i < B[g] && k < nparts; ++i) {
+        const int64_t r0 = run * nparts;
+        run += lengths[goff[g] + i];
+        const int64_t r1 = run * nparts;
+        while (k < nparts && r1 >= tot * k) {
+          const int at = (r1 - tot * k <= tot * k - r0) ? i + 1 : i;
+          cut[g].push_back(std::max(at, cut[g].back()));
+          ++k;
+        }
+      }
+      while ((int)cut[g].size() < nparts + 1) cut[g].push_back(B[g]);
+    }
+    for (int k = 0; k < nparts; ++k) {
+      std::vector<wt_config> pc;
+      std::vector<int32_t> pB, plen;
+      std::vector<float> pcm;
+      std::vector<wt_tuning> ptu;
+      std::vector<int> pg, plo;
+      for (int g = 0; g < ngroups; ++g) {
+        const int a = cut[g][k], b = cut[g][k + 1];
+        if (b <= a) continue;
+        pc.push_back(cfgs[g]);
+        pB.push_back(b - a);
+        for (int i = a; i < b; ++i) plen.push_back(lengths[goff[g] + i]);
+        pcm.push_back(cmax[g]);
+        if (tunings) ptu.push_back(tunings[g]);
+        pg.push_back(g);
+        plo.push_back(a);
+      }
+      if (pg.empty()) continue;
+      if (tunings) {  // the block choice common to every launch comes from entry 0
+        const wt_tuning &t0 = tunings[0];
+        ptu[0].tblock = t0.tblock; ptu[0].nt = t0.nt; ptu[0].cpt = t0.cpt;
+        ptu[0].a_tblock = t0.a_tblock; ptu[0].a_nt = t0.a_nt; ptu[0].a_cpt = t0.a_cpt;
+        ptu[0].ck_every = t0.ck_every;
+      }
+      BatchPlan *bp = nullptr;
+      rc = batch_build((int32_t)pg.size(), pc.data(), pB.data(), plen.data(), pcm.data(),
+                       tunings ? ptu.data() : nullptr, h->T, &bp);
+      if (rc) {
+        free_handle(h);
+        return rc;
+      }
+      h->parts.push_back(bp);
+      h->grp.push_back(pg);
+      h->lo.push_back(plo);
+    }
+  }
+  for (BatchPlan *bp : h->parts) {
+    h->soff.push_back(h->saved_bytes);
+    h->toff.push_back(h->temp_bytes);
+    h->saved_bytes += align_up(bp->saved_bytes);
+    h->temp_bytes += align_up(bp->temp_bytes);
+  }
+  if (h->parts.size() > 1) {
+    hipError_t e = hipEventCreateWithFlags(&h->fork, hipEventDisableTiming);
+    for (size_t k = 0; e == hipSuccess && k < h->parts.size(); ++k) {
+      hipStream_t st = nullptr;
+      hipEvent_t ev = nullptr;
+      e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+      if (e == hipSuccess) h->streams.push_back(st);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e == hipSuccess) h->join.push_back(ev);
+    }
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "wt_batch_plan_create: part streams");
+      free_handle(h);
+      return rc;
+    }
+  }
+  *plan_out = h;
+  return WT_OK;
+}
+
 int wt_batch_plan_destroy(void *plan) {
-  free_batch((BatchPlan *)plan);
+  free_handle((BatchHandle *)plan);
   return WT_OK;
 }
 
 int wt_batch_plan_tuning(const void *plan, int32_t group, wt_tuning *out) {
-  const BatchPlan *bp = (const BatchPlan *)plan;
-  if (!bp || !out || group < 0 || group >= bp->ngrp) {
+  const BatchHandle *h = (const BatchHandle *)plan;
+  if (!h || !out || group < 0 || group >= h->ngrp) {
     set_error("wt_batch_plan_tuning: invalid argument");
     return WT_EINVAL;
   }
-  *out = bp->g[group]->tun;
-  return WT_OK;
+  for (size_t k = 0; k < h->parts.size(); ++k)
+    for (size_t j = 0; j < h->grp[k].size(); ++j)
+      if (h->grp[k][j] == group) {
+        *out = h->parts[k]->g[j]->tun;
+        out->streams = (int32_t)h->parts.size();
+        return WT_OK;
+      }
+  set_error("wt_batch_plan_tuning: group %d has no plan", group);
+  return WT_EINVAL;
 }
 
 int wt_batch_workspace_bytes(const void *plan, int64_t *saved, int64_t *temp) {
-  const BatchPlan *bp = (const BatchPlan *)plan;
-  if (!bp) {
+  const BatchHandle *h = (const BatchHandle *)plan;
+  if (!h) {
     set_error("wt_batch_workspace_bytes: null plan");
     return WT_EINVAL;
   }
-  if (saved) *saved = (int64_t)bp->saved_bytes;
-  if (temp) *temp = (int64_t)bp->temp_bytes;
+  if (saved) *saved = (int64_t)h->saved_bytes;
+  if (temp) *temp = (int64_t)h->temp_bytes;
   return WT_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+bool handle_io_ok(const BatchHandle *h, const wt_batch_group_io *io, bool adj) {
+  if (!h || !io) return false;
+  for (int g = 0; g < h->ngrp; ++g) {
+    if (!io[g].b || !io[g].c || !io[g].x) return false;
+    if (!adj && !io[g].y) return false;
+    if (adj && !io[g].ybar) return false;
+  }
+  return true;
+}
+
+// The group arrays of part k: the caller's at the part's first instance.
+std::vector<wt_batch_group_io> part_io(const BatchHandle &h, size_t k,
+                                       const wt_batch_group_io *io) {
+  std::vector<wt_batch_group_io> v;
+  for (size_t j = 0; j < h.grp[k].size(); ++j) {
+    const int g = h.grp[k][j];
+    const int64_t a = h.lo[k][j], F = h.F[g], T = h.T, P = h.ntap[g];
+    wt_batch_group_io q = io[g];
+    q.c = io[g].c + a * F;
+    q.x = io[g].x + a * T;
+    if (q.y) q.y = io[g].y + a * T * P;
+    if (q.ybar) q.ybar = io[g].ybar + a * T * P;
+    if (q.grad_c) q.grad_c = io[g].grad_c + a * F;
+    if (q.grad_x) q.grad_x = io[g].grad_x + a * T;
+    v.push_back(q);
+  }
+  return v;
+}
+
+// Runs fn(part plan, part io, saved, temp, stream) for every part: on the
+// caller's stream for one part, else forked onto the parts' streams and
+// joined back.
+template <typename Fn>
+int run_parts(const BatchHandle &h, const wt_batch_group_io *io, char *saved, char *temp,
+              hipStream_t s, Fn fn) {
+  if (h.parts.size() == 1)
+    return fn(*h.parts[0], io, saved ? saved + h.soff[0] : nullptr, temp + h.toff[0], s);
+  WT_HIP(hipEventRecord(h.fork, s));
+  for (size_t k = 0; k < h.parts.size(); ++k) {
+    WT_HIP(hipStreamWaitEvent(h.streams[k], h.fork, 0));
+    const std::vector<wt_batch_group_io> pio = part_io(h, k, io);
+    const int rc = fn(*h.parts[k], pio.data(), saved ? saved + h.soff[k] : nullptr,
+                      temp + h.toff[k], h.streams[k]);
+    if (rc) return rc;
+    WT_HIP(hipEventRecord(h.join[k], h.streams[k]));
+  }
+  for (size_t k = 0; k < h.parts.size(); ++k) WT_HIP(hipStreamWaitEvent(s, h.join[k], 0));
+  return WT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int wt_batch_fwd(void *plan, const wt_batch_group_io *io, int32_t with_grad, void *saved,
                  void *temp, void *stream) {
-  const BatchPlan *bp = (const BatchPlan *)plan;
-  if (!batch_io_ok(bp, io, false) || !temp || (with_grad && !saved)) {
+  const BatchHandle *h = (const BatchHandle *)plan;
+  if (!handle_io_ok(h, io, false) || !temp || (with_grad && !saved)) {
     set_error("wt_batch_fwd: invalid argument");
     return WT_EINVAL;
   }
-  return batch_forward(*bp, io, with_grad != 0, (char *)saved, (char *)temp,
-                       (hipStream_t)stream);
+  const bool wg = with_grad != 0;
+  return run_parts(*h, io, (char *)saved, (char *)temp, (hipStream_t)stream,
+                   [wg](const BatchPlan &bp, const wt_batch_group_io *pio, char *sv, char *tp,
+                        hipStream_t st) { return batch_forward(bp, pio, wg, sv, tp, st); });
 }
 
 int wt_batch_adj(void *plan, const wt_batch_group_io *io, void *saved, void *temp,
                  void *stream) {
-  const BatchPlan *bp = (const BatchPlan *)plan;
-  if (!batch_io_ok(bp, io, true) || !saved || !temp) {
+  const BatchHandle *h = (const BatchHandle *)plan;
+  if (!handle_io_ok(h, io, true) || !saved || !temp) {
     set_error("wt_batch_adj: invalid argument");
     return WT_EINVAL;
   }
-  return batch_adjoint(*bp, io, (char *)saved, (char *)temp, (hipStream_t)stream);
+  return run_parts(*h, io, (char *)saved, (char *)temp, (hipStream_t)stream,
+                   [](const BatchPlan &bp, const wt_batch_group_io *pio, char *sv, char *tp,
+                      hipStream_t st) { return batch_adjoint(bp, pio, sv, tp, st); });
 }
 
 int wt_batch_fwd_adj(void *plan, const wt_batch_group_io *io, void *saved, void *temp,
                      void *stream) {
-  const BatchPlan *bp = (const BatchPlan *)plan;
-  if (!batch_io_ok(bp, io, false) || !batch_io_ok(bp, io, true) || !saved || !temp) {
+  const BatchHandle *h = (const BatchHandle *)plan;
+  if (!handle_io_ok(h, io, false) || !handle_io_ok(h, io, true) || !saved || !temp) {
     set_error("wt_batch_fwd_adj: invalid argument");
     return WT_EINVAL;
   }
-  int rc = batch_forward(*bp, io, true, (char *)saved, (char *)temp, (hipStream_t)stream);
-  if (rc) return rc;
-  return batch_adjoint(*bp, io, (char *)saved, (char *)temp, (hipStream_t)stream);
+  return run_parts(*h, io, (char *)saved, (char *)temp, (hipStream_t)stream,
+                   [](const BatchPlan &bp, const wt_batch_group_io *pio, char *sv, char *tp,
+                      hipStream_t st) {
+                     const int rc = batch_forward(bp, pio, true, sv, tp, st);
+                     return rc ? rc : batch_adjoint(bp, pio, sv, tp, st);
+                   });
 }
 
 // ---------------------------------------------------------------- profiling
