@@ -156,3 +156,32 @@ def test_batch_parts_graph_replay(cuda):
                 assert torch.equal(y0, y)
                 assert torch.equal(gc0, gc)
                 assert torch.equal(gx0, gx)
+
+
+@pytest.mark.parametrize("parts", [1, 3])
+def test_batch_instance_order(parts, cuda):
+    """Instances longest first (launches cover only the instances with steps
+    left) and in the given order (launches cover every instance of a group
+    while its longest one runs) give the same results bit for bit."""
+    items = _instances(seed=4)
+    cs = [torch.from_numpy(it[1]).to(cuda) for it in items]
+    xs = [torch.from_numpy(it[2]).to(cuda) for it in items]
+    ybars = [-torch.ones((it[0].length, 1), device=cuda) for it in items]
+    runs = []
+    for lf in (True, False):
+        bt = InstanceBatch([it[0] for it in items], cuda, tuning={"streams": parts},
+                           longest_first=lf)
+        print("case order", lf, parts, [len(g) for g in bt.groups], flush=True)
+        lens = [[bt.instances[i].length for i in idx] for idx in bt.groups]
+        if lf:
+            assert all(ln == sorted(ln, reverse=True) for ln in lens)
+        else:
+            assert any(ln != sorted(ln, reverse=True) for ln in lens)
+        bt.set_inputs(cs, xs, ybars)
+        bt.fwd_adj()
+        torch.cuda.synchronize()
+        runs.append([tuple(t.clone() for t in o) for o in bt.outputs()])
+        del bt
+    for a, b in zip(runs[0], runs[1]):
+        for ta, tb in zip(a, b):
+            assert torch.equal(ta, tb)

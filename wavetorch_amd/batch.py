@@ -39,11 +39,13 @@ class InstanceBatch:
     """Groups `instances` by configuration and owns the batch plan, the
     workspaces and padded per-group buffers on `device`.
 
-    Per group g (instances i in group order): c (B,Nx,Ny), x (B,T) with T the
+    Per group g (instances longest first, or in the given order with
+    `longest_first` False): c (B,Nx,Ny), x (B,T) with T the
     longest length of the batch (zero-padded), ybar (B,T,P), outputs y
     (B,T,P), grad_c (B,Nx,Ny), grad_x (B,T)."""
 
-    def __init__(self, instances, device, tuning=None, checkpoint_every=64):
+    def __init__(self, instances, device, tuning=None, checkpoint_every=64,
+                 longest_first=True):
         self.device = torch.device(device)
         self.instances = list(instances)
         groups = {}
@@ -54,7 +56,11 @@ class InstanceBatch:
             groups.setdefault(key, []).append(i)
         if len(groups) > 8:
             raise ValueError(f"{len(groups)} distinct configurations; at most 8 per batch")
-        self.groups = list(groups.values())
+        # longest first within a group: launches then cover only the prefix
+        # of instances that still have steps to run (in any other order they
+        # cover every instance of the group while its longest one runs)
+        self.groups = [sorted(idx, key=lambda i: -self.instances[i].length) if longest_first
+                       else idx for idx in groups.values()]
         self.where = {}
         for g, idx in enumerate(self.groups):
             for k, i in enumerate(idx):

@@ -1450,6 +1450,9 @@ struct BatchPlan {
   std::vector<Plan *> g;
   std::vector<int32_t *> d_Tlim;  // per group, per instance step count (device)
   std::vector<int> Tg;            // per group longest instance
+  std::vector<std::vector<int>> len;  // per group, per instance step count (host)
+  std::vector<char> desc;         // group's lengths non-increasing: launches cover
+                                  // only the prefix of instances with steps left
   int T = 0, K = 64, nseg = 1;
   int tb = 24, nt = 1024, R = 10, tb_a = 24, nt_a = 1024, R_a = 10;
   int RHmax = 0, RWmax = 0, RHmax_a = 0, RWmax_a = 0;
@@ -1463,6 +1466,23 @@ void free_batch(BatchPlan *bp) {
   for (int32_t *t : bp->d_Tlim)
     if (t) (void)hipFree(t);
   delete bp;
+}
+
+// Instances of group g that a launch must cover when it runs for instances
+// whose length exceeds `after` (all B unless the group's lengths are in
+// non-increasing order, when those instances are a prefix), and the useful
+// cell-steps of steps [lo, hi) over the group.
+int batch_active(const BatchPlan &bp, int g, int after) {
+  const std::vector<int> &L = bp.len[g];
+  if (!bp.desc[g]) return (int)L.size();
+  int n = 0;
+  while (n < (int)L.size() && L[n] > after) ++n;
+  return n;
+}
+int64_t batch_work(const BatchPlan &bp, int g, int lo, int hi) {
+  int64_t w = 0;
+  for (int l : bp.len[g]) w += std::max(0, std::min(l, hi) - lo);
+  return w * (int64_t)bp.g[g]->F;
 }
 
 AdjArgs base_adj(const Plan &p, const Ws &w, const float *ybar) {
@@ -1517,7 +1537,8 @@ int batch_forward(const BatchPlan &bp, const wt_batch_group_io *io, bool with_gr
       for (int g = 0; g < G; ++g) {
         out1[g] = last ? o1[g] : bufs[g][2 * pp];
         out2[g] = last ? o1[g] + (int64_t)bp.g[g]->B * bp.g[g]->F : bufs[g][2 * pp + 1];
-        if (bp.Tg[g] <= n) continue;
+        const int nact = batch_active(bp, g, n);
+        if (nact == 0) continue;
         const Plan &p = *bp.g[g];
         FwdArgs a = base_fwd(p, W[g], io[g].b, io[g].c, nullptr, io[g].x);
         a.y = io[g].y;
@@ -1527,9 +1548,9 @@ int batch_forward(const BatchPlan &bp, const wt_batch_group_io *io, bool with_gr
         a.Tlim = bp.d_Tlim[g];
         a.c2_bstride = (int)p.F;
         bl.grp[bl.ngrp] = a;
-        bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + p.B * p.ntiles;
+        bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + nact * p.ntiles;
         ++bl.ngrp;
-        work += (int64_t)p.B * p.F * k;
+        work += batch_work(bp, g, n, n + k);
       }
       {
         ProfScope ps(0, s, work);
@@ -1582,7 +1603,8 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
       BatchFwd bl{};
       int64_t work = 0;
       for (int g = 0; g < G; ++g) {
-        if (bp.Tg[g] <= n0) continue;
+        const int nact = batch_active(bp, g, n0);
+        if (nact == 0) continue;
         const Plan &p = *bp.g[g];
         FwdArgs a = base_fwd(p, W[g], io[g].b, io[g].c, nullptr, io[g].x);
         a.rec_lap = W[g].seg; a.rec_K = bp.K; a.rec_n0 = n0;
@@ -1591,9 +1613,10 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
         a.Tlim = bp.d_Tlim[g];
         a.c2_bstride = (int)p.F;
         bl.grp[bl.ngrp] = a;
-        bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + p.B * p.ntiles;
+        bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + nact * p.ntiles;
         ++bl.ngrp;
-        work += (int64_t)p.B * p.F * k;
+        // every instance with steps in the segment is recomputed to its end
+        work += (int64_t)nact * p.F * k;
       }
       {
         ProfScope ps(1, s, work);
@@ -1612,7 +1635,9 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
       BatchAdj bl{};
       int64_t work = 0;
       for (int g = 0; g < G; ++g) {
-        if (bp.Tg[g] <= n0) continue;
+        // reverse steps m - k + 1 .. m of instances with length >= m - k + 1
+        const int nact = batch_active(bp, g, m - k);
+        if (nact == 0) continue;
         const Plan &p = *bp.g[g];
         AdjArgs aa = base_adj(p, W[g], io[g].ybar);
         aa.in1 = lam[g][0]; aa.in2 = lam[g][1]; aa.out1 = lam[g][2]; aa.out2 = lam[g][3];
@@ -1620,9 +1645,9 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
         aa.Tlim = bp.d_Tlim[g];
         aa.c2_bstride = (int)p.F;
         bl.grp[bl.ngrp] = aa;
-        bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + p.B * p.ntiles_a;
+        bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + nact * p.ntiles_a;
         ++bl.ngrp;
-        work += (int64_t)p.B * p.F * k;
+        work += batch_work(bp, g, m - k, m);
       }
       {
         ProfScope ps(2, s, work);
@@ -1693,6 +1718,8 @@ static int batch_build(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
       tg = std::max(tg, (int)lengths[off + i]);
     }
     bp->Tg.push_back(tg);
+    bp->len.emplace_back(lengths + off, lengths + off + B[g]);
+    bp->desc.push_back(std::is_sorted(bp->len.back().rbegin(), bp->len.back().rend()));
     bp->T = std::max(bp->T, tg);
     off += B[g];
   }
