@@ -164,7 +164,7 @@ typedef struct wt_tuning {
   /* adjoint kernel choice; 0 = same as the forward fields above */
   int32_t a_strips, a_tcols, a_tblock, a_nt, a_cpt;
   /* batched instances: the batch runs as this many concurrent parts on as
-     many streams (0 = library choice, at most 4); single plans ignore it */
+     many streams (0 = library choice, at most 8); single plans ignore it */
   int32_t streams;
 } wt_tuning;
 
@@ -325,13 +325,19 @@ typedef struct wt_batch_group_io {
  * tblock/nt/cpt/a_tblock/a_nt/a_cpt/ck_every/streams are taken from entry 0
  * (common to every launch) and strips/tcols (and a_strips/a_tcols) per
  * group.  Only the linear cell with h = 1 is batched.
+ * A group whose lengths are in non-increasing order (longest first) has its
+ * launches cover only the instances that still have steps to run; in any
+ * other order they cover every instance while the group's longest one runs
+ * (same results, more idle workgroups).
  * The plan may split the batch into `streams` parts (contiguous instance
- * ranges of every group, balanced by steps; by default one part per ~768
- * work items of a launch, at most 3): each part runs on a stream of the
+ * ranges of every group, balanced by steps; by default one part per ~512
+ * work items of a launch, at most 4): each part runs on a stream of the
  * plan's own, forked from the caller's stream and joined back to it with
  * events, so a captured graph holds all of them and results are those of
  * the single sequence (the instances are independent).  Launches of the
- * parts overlap, filling the CUs one part's launch leaves idle at its end.
+ * parts overlap, filling the CUs that one part's launches leave idle; with
+ * groups longest first the parts hold instances of different lengths, so
+ * they reach their short-lived tails at different times.
  * Calls with one plan must not overlap (the part streams are the plan's). */
 int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
                          const int32_t *lengths, const float *cmax,

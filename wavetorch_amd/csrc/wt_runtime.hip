@@ -1833,14 +1833,16 @@ static void free_handle(BatchHandle *h) {
   delete h;
 }
 
-// Library choice of the number of parts: one per ~768 work items of the
-// first launch (three rounds of 256 workgroups), at most 3 (measured on the
-// 256-instance lens batch, ~2400 work items: 1, 2, 3, 4 parts took 130.3,
-// 118.3, 113.1, 116.1 ms).
+// Library choice of the number of parts: one per ~512 work items of the
+// first launch (two rounds of 256 workgroups), at most 4.  Measured on a
+// batch of 256 instances (sides 64-256, 1000-4000 steps, ~2400 work items,
+// instances longest first), graph replay of forward + adjoint: 1, 2, 3, 4,
+// 5, 6, 8 parts took 125.0, 118.2, 115.7, 113.2, 115.3, 116.0, 116.4 ms;
+// a 256-work-item batch on two streams was slower than on one.
 static int default_parts(const BatchPlan &bp) {
   int64_t items = 0;
   for (const Plan *p : bp.g) items += (int64_t)p->B * p->ntiles;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(3, items / 768));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(4, items / 512));
 }
 
 int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
@@ -1862,7 +1864,7 @@ int wt_batch_plan_create(int32_t ngroups, const wt_config *cfgs, const int32_t *
     h->ntap.push_back(p->ntap);
     h->F.push_back((int64_t)p->F);
   }
-  const int nparts = (tunings && tunings[0].streams > 0) ? std::min(tunings[0].streams, 4)
+  const int nparts = (tunings && tunings[0].streams > 0) ? std::min(tunings[0].streams, 8)
                                                          : default_parts(*whole);
   if (nparts <= 1) {
     h->parts.push_back(whole);
