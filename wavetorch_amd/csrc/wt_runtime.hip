@@ -1453,6 +1453,8 @@ struct BatchPlan {
   std::vector<std::vector<int>> len;  // per group, per instance step count (host)
   std::vector<char> desc;         // group's lengths non-increasing: launches cover
                                   // only the prefix of instances with steps left
+  std::vector<int> ord_f, ord_a;  // groups in a launch's table: largest regions
+                                  // first (forward, adjoint tilings)
   int T = 0, K = 64, nseg = 1;
   int tb = 24, nt = 1024, R = 10, tb_a = 24, nt_a = 1024, R_a = 10;
   int RHmax = 0, RWmax = 0, RHmax_a = 0, RWmax_a = 0;
@@ -1534,7 +1536,8 @@ int batch_forward(const BatchPlan &bp, const wt_batch_group_io *io, bool with_gr
       BatchFwd bl{};
       int64_t work = 0;
       std::vector<float *> out1(G), out2(G);
-      for (int g = 0; g < G; ++g) {
+      for (int j = 0; j < G; ++j) {
+        const int g = bp.ord_f[j];
         out1[g] = last ? o1[g] : bufs[g][2 * pp];
         out2[g] = last ? o1[g] + (int64_t)bp.g[g]->B * bp.g[g]->F : bufs[g][2 * pp + 1];
         const int nact = batch_active(bp, g, n);
@@ -1602,7 +1605,8 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
       const int k = std::min(bp.tb, n1 - n);
       BatchFwd bl{};
       int64_t work = 0;
-      for (int g = 0; g < G; ++g) {
+      for (int j = 0; j < G; ++j) {
+        const int g = bp.ord_f[j];
         const int nact = batch_active(bp, g, n0);
         if (nact == 0) continue;
         const Plan &p = *bp.g[g];
@@ -1634,7 +1638,8 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
       const int k = std::min(bp.tb_a, m - n0);
       BatchAdj bl{};
       int64_t work = 0;
-      for (int g = 0; g < G; ++g) {
+      for (int j = 0; j < G; ++j) {
+        const int g = bp.ord_a[j];
         // reverse steps m - k + 1 .. m of instances with length >= m - k + 1
         const int nact = batch_active(bp, g, m - k);
         if (nact == 0) continue;
@@ -1794,6 +1799,21 @@ static int batch_build(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
   }
   bp->saved_bytes = soff;
   bp->temp_bytes = toff;
+  // a launch's workgroups start in table order: the groups with the largest
+  // regions (the longest-running work items) first, so that the small ones
+  // fill the CUs at the launch's end
+  for (int g = 0; g < ngroups; ++g) {
+    bp->ord_f.push_back(g);
+    bp->ord_a.push_back(g);
+  }
+  auto area = [bp](int g, bool adj) {
+    const Plan *p = bp->g[g];
+    return adj ? (int64_t)p->RHmax_a * p->RWmax_a : (int64_t)p->RHmax * p->RWmax;
+  };
+  std::stable_sort(bp->ord_f.begin(), bp->ord_f.end(),
+                   [&](int a, int b) { return area(a, false) > area(b, false); });
+  std::stable_sort(bp->ord_a.begin(), bp->ord_a.end(),
+                   [&](int a, int b) { return area(a, true) > area(b, true); });
   *plan_out = bp;
   return WT_OK;
 }
