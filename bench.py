@@ -620,7 +620,15 @@ def run_batch(args, dev):
     # inside the graph) of the same batch as ONE part, so that each launch
     # runs alone on the chip: with several parts the launches overlap and an
     # event pair on a part's stream also times the wait for CUs held by the
-    # other parts' launches
+    # other parts' launches.  The timed plan's workspaces go first (a plan
+    # that keeps the whole record can hold a large share of HBM).
+    bt_tuning = bt.tuning
+    bt_groups = [{"side": bt.instances[idx[0]].cfg.Nx, "instances": len(idx),
+                  "strips": bt.tuning[g]["strips"], "tcols": bt.tuning[g]["tcols"]}
+                 for g, idx in enumerate(bt.groups)]
+    bt_ws = (bt.saved.numel() + bt.temp.numel()) / 2 ** 30
+    del bt
+    torch.cuda.empty_cache()
     bt1 = InstanceBatch(insts, dev, tuning=dict(tuning or {}, streams=1))
     bt1.set_inputs(cs, xs, ybars)
     with torch.cuda.stream(s):
@@ -651,8 +659,8 @@ def run_batch(args, dev):
         log(f"[bench] class {c} {names[c]}: {n[c]} launches, {ms[c]:.2f} ms, "
             f"{ms[c] / max(n[c], 1) * 1e3:.1f} us avg")
     log(f"[bench] graph replay {dt * 1e3:.2f} ms, eager {dt_eager * 1e3:.2f} ms "
-        f"({bt.tuning[0].get('streams', 1)} parts); one part: eager {dt_eager1 * 1e3:.2f} ms "
-        f"with HIP events")
+        f"({bt_tuning[0].get('streams', 1)} parts, workspaces {bt_ws:.1f} GiB); one part: "
+        f"eager {dt_eager1 * 1e3:.2f} ms with HIP events")
     out = {
         "metric": wl["metric"], "value": round(cells / dt / 1e9, 3), "unit": "Gcell-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -660,17 +668,15 @@ def run_batch(args, dev):
         "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded sizes, lengths and rho, see wavetorch_amd/workloads.py)",
         "config": {"workload": f"{args.workload}: {wl['desc']}", "instances": len(insts),
-                   "groups": [{"side": bt.instances[idx[0]].cfg.Nx, "instances": len(idx),
-                               "strips": bt.tuning[g]["strips"], "tcols": bt.tuning[g]["tcols"]}
-                              for g, idx in enumerate(bt.groups)],
-                   "cell_steps": cells, "tuning": bt.tuning[0]},
+                   "groups": bt_groups, "cell_steps": cells, "tuning": bt_tuning[0],
+                   "workspace_gib": round(bt_ws, 2)},
         # the dominant kernel's launches in the one-part eager run (the
         # counter summary covers the same dispatches, the last of the run)
         "roofline": roofline(args.workload, names[cls], bt1.tuning[0], cls, per_launch, avg_s,
                              nl, {"measured_on": "the batch as one part, eager, HIP events",
                                   "eager_one_part_ms": round(dt_eager1 * 1e3, 3),
                                   "eager_ms": round(dt_eager * 1e3, 3),
-                                  "parts": bt.tuning[0].get("streams", 1)}),
+                                  "parts": bt_tuning[0].get("streams", 1)}),
         "step_roofline": {"bytes_per_cell_step": wl["bytes"],
                           "basis": "algorithmic bytes (not a bound)",
                           "achieved": round(wl["bytes"] * cells / dt / 1e9, 1), "unit": "GB/s",
@@ -688,7 +694,7 @@ def run_batch(args, dev):
             bb.copy_(a)
         torch.cuda.synchronize()
         del a, bb
-    del bt, bt1
+    del bt1
     if not args.no_plain:
         # the plain per-instance path: one plan per instance, one launch per
         # step, instance after instance (plans made before the clock starts)

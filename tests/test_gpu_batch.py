@@ -48,10 +48,10 @@ def _oracle(inst, c, x, ybar):
     return yo[0], g["gc"], g["gx"][0]
 
 
-@pytest.mark.parametrize("tuning,K", [(None, 64), (None, 32),
+@pytest.mark.parametrize("tuning,K", [(None, 64), (None, 32), (None, 300),
                                       ({"tblock": 8, "nt": 1024, "cpt": 8, "a_tblock": 6,
                                         "a_nt": 1024, "a_cpt": 6, "strips": 2, "tcols": 2}, 40)],
-                         ids=["auto-K64", "auto-K32", "tiles2x2-K40"])
+                         ids=["auto-K64", "auto-K32", "auto-K300-one-segment", "tiles2x2-K40"])
 def test_batch_matches_oracle(tuning, K, cuda):
     items = _instances()
     bt = InstanceBatch([it[0] for it in items], cuda, tuning=tuning, checkpoint_every=K)

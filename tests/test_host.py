@@ -478,3 +478,25 @@ def test_holey_oracle_closed_forms():
             args_m[k][i] -= eps
             fd = (J(*args_p) - J(*args_m)).item() / (2 * eps)
             assert abs(p64[k].grad[i].item() - fd) <= 1e-6 * max(1.0, abs(fd)), (k, i)
+
+
+def test_batch_candidates_one_segment_within_budget():
+    """One-segment candidates (K >= the longest length, a multiple of both
+    depths) are offered only when their record fits the budget."""
+    from wavetorch_amd import autotune
+    from wavetorch_amd.batch import Instance
+    cfg = LoopConfig(Nx=40, Ny=36, dt=0.707, h=1.0, src_px=((9, 18, 0),), taps=((30, 12, 1),))
+    insts = [Instance(cfg, torch.zeros(40, 36), T, 1.0) for T in (150, 260, 97)]
+    assert autotune.batch_record_bytes(insts, 10) == 3 * 4 * 10 * 40 * 37
+    base = autotune.batch_candidates(insts, 0)
+    assert base == autotune.BATCH_CANDIDATES
+    big = autotune.batch_candidates(insts, float("inf"))
+    extra = big[len(base):]
+    assert extra and all(c["ck_every"] >= 260 and c["ck_every"] % c["tblock"] == 0
+                         and c["ck_every"] % c["a_tblock"] == 0 for c in extra)
+    k = autotune._aligned_k(24, 24, kmin=260)
+    assert k == 264
+    mid = autotune.batch_candidates(insts, autotune.batch_record_bytes(insts, 264))
+    assert any(c["ck_every"] == 264 for c in mid[len(base):])
+    assert all(autotune.batch_record_bytes(insts, c["ck_every"])
+               <= autotune.batch_record_bytes(insts, 264) for c in mid[len(base):])

@@ -311,6 +311,29 @@ BATCH_CANDIDATES = [
 ]
 
 
+def batch_record_bytes(insts, K):
+    """Device bytes of the saved record of one segment of K steps (lap u of
+    every cell and step, rows of Ny + 1) over the batch's instances."""
+    return sum(4 * K * i.cfg.Nx * (i.cfg.Ny + 1) for i in insts)
+
+
+def batch_candidates(insts, max_record_bytes):
+    """BATCH_CANDIDATES plus, for each block pair, one segment over the whole
+    batch (K = the smallest multiple of both depths >= the longest length):
+    the forward then records every step and the reverse sweep recomputes
+    nothing, for a record of batch_record_bytes(insts, K) -- offered only
+    within `max_record_bytes`."""
+    T = max(i.length for i in insts)
+    out = list(BATCH_CANDIDATES)
+    pairs = sorted({(c["tblock"], c["cpt"], c["a_tblock"], c["a_cpt"]) for c in BATCH_CANDIDATES})
+    for tb, r, atb, ar in pairs:
+        k = _aligned_k(tb, atb, kmin=T)
+        if batch_record_bytes(insts, k) <= max_record_bytes:
+            out.append({"tblock": tb, "nt": 1024, "cpt": r, "a_tblock": atb, "a_nt": 1024,
+                        "a_cpt": ar, "ck_every": k})
+    return out
+
+
 def batch_key(insts, arch=None):
     """Cache key of a batch: the sorted (Nx, Ny, length) of its instances."""
     import hashlib
@@ -325,14 +348,20 @@ def cached_batch(insts, arch=None):
     return dict(e["tuning"]) if e else None
 
 
-def tune_batch(insts, cs, xs, ybars, device, candidates=None, reps=2, verbose=False):
+def tune_batch(insts, cs, xs, ybars, device, candidates=None, reps=2, verbose=False,
+               mem_fraction=0.5):
     """Times the forward + adjoint of the whole batch (wt_batch_fwd_adj) for
     each candidate block variant / time-block depth / checkpoint interval (the
     per-group tiling follows from each: the largest square tiles that fit) and
-    caches the fastest under batch_key(insts)."""
+    caches the fastest under batch_key(insts).  Without `candidates`:
+    batch_candidates with a record of at most `mem_fraction` of the device's
+    memory."""
     from .batch import InstanceBatch
+    if candidates is None:
+        total = torch.cuda.get_device_properties(torch.device(device)).total_memory
+        candidates = batch_candidates(insts, mem_fraction * total)
     best, best_t = None, float("inf")
-    for cand in candidates or BATCH_CANDIDATES:
+    for cand in candidates:
         try:
             bt = InstanceBatch(insts, device, tuning=cand)
         except (ValueError, RuntimeError) as e:
@@ -352,6 +381,7 @@ def tune_batch(insts, cs, xs, ybars, device, candidates=None, reps=2, verbose=Fa
         if t < best_t:
             best, best_t = dict(cand), t
         del bt
+        torch.cuda.empty_cache()
     if best is not None:
         with _lock:
             d = _load()

@@ -220,10 +220,10 @@ __device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
   extern __shared__ float smem[];
   const int tid = threadIdx.x;
   const int b = w / a.ntiles, ti = w - (w / a.ntiles) * a.ntiles;
-  // a sample whose own length ends before this launch has nothing to do; a
-  // recompute pass keeps going to the end of the segment holding the
-  // sample's last step, so that the adjoint finds every record it reads
-  if (a.Tlim && (REC ? a.rec_n0 : a.n0) >= a.Tlim[b]) return;
+  // a sample whose own length ends before this launch has nothing to do (a
+  // recording launch too: the adjoint reads no record of a step at or past
+  // the sample's length)
+  if (a.Tlim && a.n0 >= a.Tlim[b]) return;
   const int32_t *td = a.tiles + ti * TILE_W;
   const CsGeom g = cs_geom<R, true>(td, tid);
   const int NBUF = (g.RW + 2) * g.CS;
@@ -564,6 +564,11 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
   }
   const uint32_t voff = cs_out_voff(g, row0, Ny);
   const int64_t out_bytes = (int64_t)(g.or1 - g.or0) * Ny * 4;
+  // record of step j (read at reverse step m = j + 1): a sample's steps at or
+  // past its length are not recorded, and lambda_m is exactly 0 there, so
+  // those reads are sent out of range (they return 0)
+  const int Lb = a.Tlim ? a.Tlim[b] : 0x7fffffff;
+  auto rec_off = [&](int j) { return j < Lb ? voff : 0x80000000u; };
   __syncthreads();
 
   auto seg_rsrc = [&](int m) {
@@ -575,13 +580,14 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
   // SG1: odd steps)
   {
     const __amdgpu_buffer_rsrc_t rs0 = seg_rsrc(a.m_hi);
+    const uint32_t voff0 = rec_off(a.m_hi - 1);
 #pragma unroll
-    for (int r = 0; r < R; ++r) SG0[r] = cs_rec_load(rs0, voff + (uint32_t)(r * Ny * 4));
+    for (int r = 0; r < R; ++r) SG0[r] = cs_rec_load(rs0, voff0 + (uint32_t)(r * Ny * 4));
     // issued even for a one-step launch (out of range, reads nothing) so
     // that the loop's wait counts need not cover a path without them
     const bool two = a.nsteps > 1;
     const __amdgpu_buffer_rsrc_t rs1 = seg_rsrc(two ? a.m_hi - 1 : a.m_hi);
-    const uint32_t voff1 = two ? voff : 0x80000000u;
+    const uint32_t voff1 = two ? rec_off(a.m_hi - 2) : 0x80000000u;
 #pragma unroll
     for (int r = 0; r < R; ++r) SG1[r] = cs_rec_load(rs1, voff1 + (uint32_t)(r * Ny * 4));
   }
@@ -595,7 +601,7 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
     // record index m-3-seg_n0 >= 0: m-2 >= m_hi-nsteps+1 >= seg_n0+1
     const bool prefetch = s + 2 < a.nsteps;
     const __amdgpu_buffer_rsrc_t rsn = seg_rsrc(prefetch ? m - 2 : m);
-    const uint32_t pvoff = prefetch ? voff : 0x80000000u;
+    const uint32_t pvoff = prefetch ? rec_off(m - 3) : 0x80000000u;
     float *gsrc_row = a.gsrc + ((int64_t)b * a.T + (m - 1)) * a.nsrcslot;
     if (need) {
       if (wave_sp) {

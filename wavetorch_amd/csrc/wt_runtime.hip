@@ -384,9 +384,9 @@ static int plan_layout(Plan &p) {
   p.saved_yraw = align_up((size_t)p.B * p.T * std::max(p.ntap, 1) * sizeof(float));
   // the record of the last segment, written by the forward itself: the
   // reverse sweep starts there and needs no recompute for it (one segment in
-  // all when K >= T).  Batched plans keep their records in temp only.
+  // all when K >= T)
   const size_t segcells = (size_t)p.B * p.K * p.Nx * (p.Ny + 1);
-  p.saved_rec = p.c2ps ? 0 : align_up((nlc ? 2 : 1) * segcells * sizeof(float));
+  p.saved_rec = align_up((nlc ? 2 : 1) * segcells * sizeof(float));
   p.saved_bytes = p.saved_coef + p.saved_ck + p.saved_yraw + p.saved_rec;
   // temp
   p.t_state = align_up(8 * BF);  // 4 state pairs (ping-pong + segment in/out)
@@ -1529,6 +1529,9 @@ int batch_forward(const BatchPlan &bp, const wt_batch_group_io *io, bool with_gr
       i1[g] = c1[g];
       i2[g] = c2[g];
     }
+    // the last segment also records lap(u) of its steps into the saved
+    // workspace (the reverse sweep starts there without a recompute)
+    const bool rec = with_grad && sg + 1 == bp.nseg;
     int pp = 0;
     for (int n = n0; n < n1;) {
       const int k = std::min(bp.tb, n1 - n);
@@ -1546,6 +1549,9 @@ int batch_forward(const BatchPlan &bp, const wt_batch_group_io *io, bool with_gr
         FwdArgs a = base_fwd(p, W[g], io[g].b, io[g].c, nullptr, io[g].x);
         a.y = io[g].y;
         a.yraw = with_grad ? W[g].yraw : nullptr;
+        if (rec) {
+          a.rec_lap = W[g].rec; a.rec_K = bp.K; a.rec_n0 = n0;
+        }
         a.in1 = i1[g]; a.in2 = i2[g]; a.out1 = out1[g]; a.out2 = out2[g];
         a.n0 = n; a.nsteps = k;
         a.Tlim = bp.d_Tlim[g];
@@ -1556,8 +1562,8 @@ int batch_forward(const BatchPlan &bp, const wt_batch_group_io *io, bool with_gr
         work += batch_work(bp, g, n, n + k);
       }
       {
-        ProfScope ps(0, s, work);
-        WT_HIP(launch_fwd_cs_batch(bp.nt, bp.R, false, bl, bp.RHmax, bp.RWmax, s));
+        ProfScope ps(rec ? 1 : 0, s, work);
+        WT_HIP(launch_fwd_cs_batch(bp.nt, bp.R, rec, bl, bp.RHmax, bp.RWmax, s));
       }
       for (int g = 0; g < G; ++g) {
         i1[g] = out1[g];
@@ -1593,7 +1599,9 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
   }
   for (int sg = bp.nseg - 1; sg >= 0; --sg) {
     const int n0 = sg * bp.K, n1 = std::min(bp.T, n0 + bp.K);
-    // recompute the segment from its checkpoint, recording lap(u_n)
+    // recompute the segment from its checkpoint, recording lap(u_n) (the
+    // last segment's record is the forward's)
+    const int nrec = sg + 1 == bp.nseg ? n0 : n1;
     std::vector<const float *> i1(G), i2(G);
     for (int g = 0; g < G; ++g) {
       const int64_t BF = (int64_t)bp.g[g]->B * bp.g[g]->F;
@@ -1601,13 +1609,13 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
       i2[g] = i1[g] + BF;
     }
     int pp = 0;
-    for (int n = n0; n < n1;) {
+    for (int n = n0; n < nrec;) {
       const int k = std::min(bp.tb, n1 - n);
       BatchFwd bl{};
       int64_t work = 0;
       for (int j = 0; j < G; ++j) {
         const int g = bp.ord_f[j];
-        const int nact = batch_active(bp, g, n0);
+        const int nact = batch_active(bp, g, n);
         if (nact == 0) continue;
         const Plan &p = *bp.g[g];
         FwdArgs a = base_fwd(p, W[g], io[g].b, io[g].c, nullptr, io[g].x);
@@ -1619,8 +1627,7 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
         bl.grp[bl.ngrp] = a;
         bl.start[bl.ngrp + 1] = bl.start[bl.ngrp] + nact * p.ntiles;
         ++bl.ngrp;
-        // every instance with steps in the segment is recomputed to its end
-        work += (int64_t)nact * p.F * k;
+        work += batch_work(bp, g, n, n + k);
       }
       {
         ProfScope ps(1, s, work);
