@@ -297,7 +297,7 @@ def summarize_pmc(fetch_dir, write_dir, calib_bytes, extra_dirs=()):
     def keep(k, pairs):
         vals = [v for _, v in sorted(pairs)]
         n = tail.get(short(k))
-        return vals[-n:] if n else vals
+        return vals if n is None else vals[max(len(vals) - n, 0):]
 
     def read(d, counter):
         agg = collections.defaultdict(list)
@@ -306,7 +306,7 @@ def summarize_pmc(fetch_dir, write_dir, calib_bytes, extra_dirs=()):
                 if r["Counter_Name"] == counter:
                     agg[r["Kernel_Name"]].append((int(r.get("Dispatch_Id") or 0),
                                                   float(r["Counter_Value"])))
-        return {k: keep(k, v) for k, v in agg.items()}
+        return {k: kv for k, kv in ((k, keep(k, v)) for k, v in agg.items()) if kv}
 
     def read_all(d):
         agg = collections.defaultdict(lambda: collections.defaultdict(list))
@@ -314,7 +314,12 @@ def summarize_pmc(fetch_dir, write_dir, calib_bytes, extra_dirs=()):
             for r in csv.DictReader(open(f)):
                 agg[r["Kernel_Name"]][r["Counter_Name"]].append(
                     (int(r.get("Dispatch_Id") or 0), float(r["Counter_Value"])))
-        return {k: {cn: keep(k, v) for cn, v in c.items()} for k, c in agg.items()}
+        out = {}
+        for k, c in agg.items():
+            kc = {cn: kv for cn, kv in ((cn, keep(k, v)) for cn, v in c.items()) if kv}
+            if kc:
+                out[k] = kc
+        return out
 
     fa, wa = read(fetch_dir, "FETCH_SIZE"), read(write_dir, "WRITE_SIZE")
     fe = {k: sum(v) / len(v) for k, v in fa.items()}
