@@ -313,7 +313,9 @@ BATCH_CANDIDATES = [
 
 def batch_record_bytes(insts, K):
     """Device bytes of the saved record of one segment of K steps (lap u of
-    every cell and step, rows of Ny + 1) over the batch's instances."""
+    every cell and step, rows of Ny + 1) over the batch's instances, at K
+    step slots per instance (an upper bound: a one-segment record is packed
+    by each instance's own length)."""
     return sum(4 * K * i.cfg.Nx * (i.cfg.Ny + 1) for i in insts)
 
 

@@ -317,6 +317,10 @@ __device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
   }
   const uint32_t voff = cs_out_voff(g, row0, Ny);
   const int64_t out_bytes = (int64_t)(g.or1 - g.or0) * Ny * 4;
+  // record slots of this sample; steps at or past its length are not
+  // recorded (empty slice: the stores are dropped)
+  const int64_t rslot0 = a.rec_soff ? (int64_t)a.rec_soff[b] : (int64_t)b * a.rec_K;
+  const int Lb = a.Tlim ? a.Tlim[b] : 0x7fffffff;
   __syncthreads();
 
   auto step = [&](const float *P, float *Q, float(&Uc)[R], float(&Up)[R], int s) {
@@ -325,10 +329,11 @@ __device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
     cone(g, s, a.nsteps, lo, hi);
     const bool need = g.active && row0 < hi && row0 + R > lo;
     // the (sample, step) slice of the record, based at the first output row
-    const int64_t ro = ((int64_t)b * a.rec_K + (n - a.rec_n0)) * F + (int64_t)g.or0 * Ny;
-    const __amdgpu_buffer_rsrc_t rsl = cs_rsrc(REC ? a.rec_lap + ro : a.in1, REC ? out_bytes : 0);
+    const int64_t ro = (rslot0 + (n - a.rec_n0)) * F + (int64_t)g.or0 * Ny;
+    const int64_t rbytes = n < Lb ? out_bytes : 0;
+    const __amdgpu_buffer_rsrc_t rsl = cs_rsrc(REC ? a.rec_lap + ro : a.in1, REC ? rbytes : 0);
     const __amdgpu_buffer_rsrc_t rsu =
-        cs_rsrc((REC && NL) ? a.rec_u + ro : a.in1, (REC && NL) ? out_bytes : 0);
+        cs_rsrc((REC && NL) ? a.rec_u + ro : a.in1, (REC && NL) ? rbytes : 0);
     const float xv = blk_src ? xs[s] : 0.0f;
     if (need)
       cs_fwd_rows<R, NL, REC>(a, P, Q, Uc, Up, KA, KB, KC, KD, KS, g.base, g.CS, vmask, xv, rsl,
@@ -571,10 +576,10 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
   auto rec_off = [&](int j) { return j < Lb ? voff : 0x80000000u; };
   __syncthreads();
 
+  const int64_t sslot0 = a.seg_soff ? (int64_t)a.seg_soff[b] : (int64_t)b * a.seg_K;
   auto seg_rsrc = [&](int m) {
-    return cs_rsrc(
-        a.seg_lap + ((int64_t)b * a.seg_K + (m - 1 - a.seg_n0)) * F + (int64_t)g.or0 * Ny,
-        out_bytes);
+    return cs_rsrc(a.seg_lap + (sslot0 + (m - 1 - a.seg_n0)) * F + (int64_t)g.or0 * Ny,
+                   out_bytes);
   };
   // lap(u_{m-1}) of the first two steps of this launch (SG0: even steps,
   // SG1: odd steps)

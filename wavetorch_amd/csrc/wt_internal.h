@@ -105,6 +105,9 @@ struct FwdArgs {
   // rec[(b*K + (n - rec_n0)) * Nx*(Ny+1) + r*(Ny+1) + c]
   float *rec_lap, *rec_u;
   int rec_n0, rec_K;
+  // or, when set, sample b's steps start at step slot rec_soff[b] instead of
+  // b*K (a record packed by each sample's own length)
+  const int32_t *rec_soff;
   // batched instances: per-sample step count (a sample's work items skip
   // launches past its end; NULL = all samples run T steps) and the stride of
   // the C2 field between samples (0 = one field shared by the batch)
@@ -134,6 +137,7 @@ struct AdjArgs {
   const float *yraw; // (B,T,ntap)
   const float *seg_lap, *seg_u;  // segment buffer written by the record pass
   int seg_n0, seg_K;             // seg holds steps n = seg_n0 .. seg_n0+K-1
+  const int32_t *seg_soff;       // per-sample first step slot, or NULL (b*K)
   float *gacc;       // (B,Nx,Ny) running sum of lap(u_{m-1})*lambda_m (linear)
   float *gacc_b, *gacc_rho;  // nonlinear accumulators (B,Nx,Ny)
   float *gsrc;       // (B,T,nsrcslot) lambda_m at source pixels

@@ -176,6 +176,9 @@ struct Plan {
   bool tiled = false;
   // batched instances: one c (hence C2) per sample instead of one per plan
   bool c2ps = false;
+  // step slots of the saved record (0 = B * K; batched instances in one
+  // segment: the sum of their own lengths)
+  int64_t rec_steps = 0;
   // workspace layout (bytes)
   size_t F = 0;
   size_t saved_coef = 0, saved_ck = 0, saved_yraw = 0, saved_rec = 0, saved_bytes = 0;
@@ -385,7 +388,8 @@ static int plan_layout(Plan &p) {
   // the record of the last segment, written by the forward itself: the
   // reverse sweep starts there and needs no recompute for it (one segment in
   // all when K >= T)
-  const size_t segcells = (size_t)p.B * p.K * p.Nx * (p.Ny + 1);
+  const size_t segcells =
+      (size_t)(p.rec_steps > 0 ? p.rec_steps : (int64_t)p.B * p.K) * p.Nx * (p.Ny + 1);
   p.saved_rec = align_up((nlc ? 2 : 1) * segcells * sizeof(float));
   p.saved_bytes = p.saved_coef + p.saved_ck + p.saved_yraw + p.saved_rec;
   // temp
@@ -1453,6 +1457,8 @@ struct BatchPlan {
   std::vector<std::vector<int>> len;  // per group, per instance step count (host)
   std::vector<char> desc;         // group's lengths non-increasing: launches cover
                                   // only the prefix of instances with steps left
+  std::vector<int32_t *> d_soff;  // one segment: per instance first step slot of
+                                  // the record, packed by length (device), or null
   std::vector<int> ord_f, ord_a;  // groups in a launch's table: largest regions
                                   // first (forward, adjoint tilings)
   int T = 0, K = 64, nseg = 1;
@@ -1466,6 +1472,8 @@ void free_batch(BatchPlan *bp) {
   if (!bp) return;
   for (Plan *p : bp->g) free_plan(p);
   for (int32_t *t : bp->d_Tlim)
+    if (t) (void)hipFree(t);
+  for (int32_t *t : bp->d_soff)
     if (t) (void)hipFree(t);
   delete bp;
 }
@@ -1550,7 +1558,7 @@ int batch_forward(const BatchPlan &bp, const wt_batch_group_io *io, bool with_gr
         a.y = io[g].y;
         a.yraw = with_grad ? W[g].yraw : nullptr;
         if (rec) {
-          a.rec_lap = W[g].rec; a.rec_K = bp.K; a.rec_n0 = n0;
+          a.rec_lap = W[g].rec; a.rec_K = bp.K; a.rec_n0 = n0; a.rec_soff = bp.d_soff[g];
         }
         a.in1 = i1[g]; a.in2 = i2[g]; a.out1 = out1[g]; a.out2 = out2[g];
         a.n0 = n; a.nsteps = k;
@@ -1652,6 +1660,7 @@ int batch_adjoint(const BatchPlan &bp, const wt_batch_group_io *io, char *saved,
         if (nact == 0) continue;
         const Plan &p = *bp.g[g];
         AdjArgs aa = base_adj(p, W[g], io[g].ybar);
+        aa.seg_soff = bp.d_soff[g];  // one segment: the forward's packed record
         aa.in1 = lam[g][0]; aa.in2 = lam[g][1]; aa.out1 = lam[g][2]; aa.out2 = lam[g][3];
         aa.m_hi = m; aa.nsteps = k; aa.seg_n0 = n0;
         aa.Tlim = bp.d_Tlim[g];
@@ -1778,6 +1787,17 @@ static int batch_build(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
       return rc;
     }
     p->c2ps = true;
+    // one segment: the forward records each instance's own steps only, so the
+    // record packs them by length (instance i's steps start at slot rslot[i])
+    std::vector<int32_t> rslot;
+    if (bp->nseg == 1) {
+      int64_t acc = 0;
+      for (int i = 0; i < B[g]; ++i) {
+        rslot.push_back((int32_t)acc);
+        acc += std::min(lengths[off + i], bp->K);
+      }
+      p->rec_steps = acc;
+    }
     plan_layout(*p);
     bp->g.push_back(p);
     if ((rc = plan_upload(p))) {
@@ -1791,6 +1811,18 @@ static int batch_build(int32_t ngroups, const wt_config *cfgs, const int32_t *B,
     bp->d_Tlim.push_back(d);
     if (e != hipSuccess) {
       rc = hip_fail(e, "wt_batch_plan_create: lengths table");
+      free_batch(bp);
+      return rc;
+    }
+    int32_t *ds = nullptr;
+    if (!rslot.empty()) {
+      e = hipMalloc((void **)&ds, rslot.size() * sizeof(int32_t));
+      if (e == hipSuccess)
+        e = hipMemcpy(ds, rslot.data(), rslot.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    }
+    bp->d_soff.push_back(ds);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "wt_batch_plan_create: record slot table");
       free_batch(bp);
       return rc;
     }
