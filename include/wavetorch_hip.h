@@ -329,6 +329,11 @@ typedef struct wt_batch_group_io {
  * launches cover only the instances that still have steps to run; in any
  * other order they cover every instance while the group's longest one runs
  * (same results, more idle workgroups).
+ * The forward records lap(u) of the last checkpoint segment's steps into the
+ * saved workspace, where the reverse sweep starts without a recompute; with
+ * ck_every >= the longest length (one segment) that is every step of every
+ * instance up to its own length, packed by length (about
+ * 4 * sum(L_i) * Nx * (Ny + 1) bytes per group), and nothing is recomputed.
  * The plan may split the batch into `streams` parts (contiguous instance
  * ranges of every group, balanced by steps; by default one part per ~512
  * work items of a launch, at most 4): each part runs on a stream of the
