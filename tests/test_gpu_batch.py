@@ -158,11 +158,12 @@ def test_batch_parts_graph_replay(cuda):
                 assert torch.equal(gx0, gx)
 
 
-@pytest.mark.parametrize("parts", [1, 3])
-def test_batch_instance_order(parts, cuda):
+@pytest.mark.parametrize("parts,K", [(1, 64), (3, 64), (1, 300), (3, 300)])
+def test_batch_instance_order(parts, K, cuda):
     """Instances longest first (launches cover only the instances with steps
     left) and in the given order (launches cover every instance of a group
-    while its longest one runs) give the same results bit for bit."""
+    while its longest one runs) give the same results bit for bit; K = 300 is
+    one segment, whose record the forward writes packed by instance length."""
     items = _instances(seed=4)
     cs = [torch.from_numpy(it[1]).to(cuda) for it in items]
     xs = [torch.from_numpy(it[2]).to(cuda) for it in items]
@@ -170,8 +171,8 @@ def test_batch_instance_order(parts, cuda):
     runs = []
     for lf in (True, False):
         bt = InstanceBatch([it[0] for it in items], cuda, tuning={"streams": parts},
-                           longest_first=lf)
-        print("case order", lf, parts, [len(g) for g in bt.groups], flush=True)
+                           checkpoint_every=K, longest_first=lf)
+        print("case order", lf, parts, K, [len(g) for g in bt.groups], flush=True)
         lens = [[bt.instances[i].length for i in idx] for idx in bt.groups]
         if lf:
             assert all(ln == sorted(ln, reverse=True) for ln in lens)
@@ -185,3 +186,8 @@ def test_batch_instance_order(parts, cuda):
     for a, b in zip(runs[0], runs[1]):
         for ta, tb in zip(a, b):
             assert torch.equal(ta, tb)
+    for (inst, c, x), (y, gc, gx) in zip(items, runs[0]):
+        yo, gco, gxo = _oracle(inst, c, x, -np.ones((inst.length, 1)))
+        assert np.array_equal(y.cpu().numpy(), yo), inst.length
+        assert rel(gc.cpu().numpy(), gco) < 1e-4, inst.length
+        assert rel(gx.cpu().numpy(), gxo) < 1e-4, inst.length
