@@ -213,8 +213,9 @@ __device__ unsigned long long *wt_phase_ts = nullptr;
 
 // Work item w (sample w / ntiles, tile w % ntiles) of the forward launch
 // described by `a`; shared by the single-configuration kernel and the
-// batched kernel (one launch over several configurations).
-template <int NT, int R, int NL, bool REC>
+// batched kernel (one launch over several configurations; BATCH compiles in
+// the per-sample lengths and record slots).
+template <int NT, int R, int NL, bool REC, bool BATCH>
 __device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
   WT_TS(0);
   extern __shared__ float smem[];
@@ -223,7 +224,7 @@ __device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
   // a sample whose own length ends before this launch has nothing to do (a
   // recording launch too: the adjoint reads no record of a step at or past
   // the sample's length)
-  if (a.Tlim && a.n0 >= a.Tlim[b]) return;
+  if (BATCH && a.Tlim && a.n0 >= a.Tlim[b]) return;
   const int32_t *td = a.tiles + ti * TILE_W;
   const CsGeom g = cs_geom<R, true>(td, tid);
   const int NBUF = (g.RW + 2) * g.CS;
@@ -319,8 +320,9 @@ __device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
   const int64_t out_bytes = (int64_t)(g.or1 - g.or0) * Ny * 4;
   // record slots of this sample; steps at or past its length are not
   // recorded (empty slice: the stores are dropped)
-  const int64_t rslot0 = a.rec_soff ? (int64_t)a.rec_soff[b] : (int64_t)b * a.rec_K;
-  const int Lb = a.Tlim ? a.Tlim[b] : 0x7fffffff;
+  const int64_t rslot0 =
+      (BATCH && a.rec_soff) ? (int64_t)a.rec_soff[b] : (int64_t)b * a.rec_K;
+  const int Lb = (BATCH && a.Tlim) ? a.Tlim[b] : 0x7fffffff;
   __syncthreads();
 
   auto step = [&](const float *P, float *Q, float(&Uc)[R], float(&Up)[R], int s) {
@@ -380,7 +382,7 @@ __device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
 
 template <int NT, int R, int NL, bool REC>
 __global__ __launch_bounds__(NT) void k_fwd_cs(FwdArgs a) {
-  fwd_cs_body<NT, R, NL, REC>(a, xcd_remap(blockIdx.x, gridDim.x));
+  fwd_cs_body<NT, R, NL, REC, false>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Group of a batched launch that work item w belongs to.
@@ -395,7 +397,7 @@ template <int NT, int R, int NL, bool REC>
 __global__ __launch_bounds__(NT) void k_fwd_cs_batch(BatchFwd bl) {
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int gi = batch_group(bl.start, bl.ngrp, w);
-  fwd_cs_body<NT, R, NL, REC>(bl.grp[gi], w - bl.start[gi]);
+  fwd_cs_body<NT, R, NL, REC, true>(bl.grp[gi], w - bl.start[gi]);
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -450,15 +452,16 @@ __device__ __forceinline__ void cs_adj_rows(
   }
 }
 
-// Linear cell.  LDS holds w*lambda.  Work item w of the launch `a`.
-template <int NT, int R>
+// Linear cell.  LDS holds w*lambda.  Work item w of the launch `a` (BATCH:
+// the batched kernel, with per-sample lengths and record slots).
+template <int NT, int R, bool BATCH>
 __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
   WT_TS(0);
   extern __shared__ float smem[];
   const int tid = threadIdx.x;
   const int b = w / a.ntiles, ti = w - (w / a.ntiles) * a.ntiles;
   // lambda_m = 0 for every m of this launch when the sample ends before it
-  if (a.Tlim && a.m_hi - a.nsteps + 1 > a.Tlim[b]) return;
+  if (BATCH && a.Tlim && a.m_hi - a.nsteps + 1 > a.Tlim[b]) return;
   const int32_t *td = a.tiles + ti * TILE_W;
   const CsGeom g = cs_geom<R>(td, tid);
   const int NBUF = (g.RW + 2) * g.CS;
@@ -572,11 +575,12 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
   // record of step j (read at reverse step m = j + 1): a sample's steps at or
   // past its length are not recorded, and lambda_m is exactly 0 there, so
   // those reads are sent out of range (they return 0)
-  const int Lb = a.Tlim ? a.Tlim[b] : 0x7fffffff;
-  auto rec_off = [&](int j) { return j < Lb ? voff : 0x80000000u; };
+  const int Lb = (BATCH && a.Tlim) ? a.Tlim[b] : 0x7fffffff;
+  auto rec_off = [&](int j) { return (!BATCH || j < Lb) ? voff : 0x80000000u; };
   __syncthreads();
 
-  const int64_t sslot0 = a.seg_soff ? (int64_t)a.seg_soff[b] : (int64_t)b * a.seg_K;
+  const int64_t sslot0 =
+      (BATCH && a.seg_soff) ? (int64_t)a.seg_soff[b] : (int64_t)b * a.seg_K;
   auto seg_rsrc = [&](int m) {
     return cs_rsrc(a.seg_lap + (sslot0 + (m - 1 - a.seg_n0)) * F + (int64_t)g.or0 * Ny,
                    out_bytes);
@@ -677,14 +681,14 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
 
 template <int NT, int R>
 __global__ __launch_bounds__(NT) void k_adj_cs(AdjArgs a) {
-  adj_cs_body<NT, R>(a, xcd_remap(blockIdx.x, gridDim.x));
+  adj_cs_body<NT, R, false>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 template <int NT, int R>
 __global__ __launch_bounds__(NT) void k_adj_cs_batch(BatchAdj bl) {
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   const int gi = batch_group(bl.start, bl.ngrp, w);
-  adj_cs_body<NT, R>(bl.grp[gi], w - bl.start[gi]);
+  adj_cs_body<NT, R, true>(bl.grp[gi], w - bl.start[gi]);
 }
 
 // ------------------------------------------------------------------ nonlinear adjoint
