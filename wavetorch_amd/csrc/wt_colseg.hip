@@ -133,6 +133,29 @@ __device__ __forceinline__ uint32_t cs_out_voff(const CsGeom &g, int row0, int N
   return (uint32_t)(((g.R0 + row0 - g.or0) * Ny + gc) * 4);
 }
 
+// Zeroes LDS buffer b1 and, of b0, only the cells that the workgroup's fill
+// of b0 (every cell of every segment of the region's columns) leaves alone:
+// the ring columns 0 and RW+1 and, in the region's columns, the ring row 0
+// and the rows past the last segment (CS - nrows cells per column).
+__device__ __forceinline__ void cs_zero_lds(float *b0, float *b1, const CsGeom &g, int nrows,
+                                            int NBUF) {
+  for (int i = threadIdx.x; i < NBUF; i += blockDim.x) b1[i] = 0.0f;
+  const int tail = g.CS - nrows;
+  const int nring = 2 * g.CS + g.RW * tail;
+  for (int i = threadIdx.x; i < nring; i += blockDim.x) {
+    int idx;
+    if (i < g.CS) {
+      idx = i;
+    } else if (i < 2 * g.CS) {
+      idx = (g.RW + 1) * g.CS + (i - g.CS);
+    } else {
+      const int j = i - 2 * g.CS, col = j / tail, r = j - col * tail;
+      idx = (col + 1) * g.CS + (r == 0 ? 0 : nrows + r);
+    }
+    b0[idx] = 0.0f;
+  }
+}
+
 // ------------------------------------------------------------------ forward
 
 // Writes the output tile of the field in LDS buffer Q to fields[(b,idx)]:
@@ -272,8 +295,8 @@ __device__ __forceinline__ void fwd_cs_body(const FwdArgs &a, const int w) {
     const int32_t *t = a.taps + (td[10] + tid) * TAP_W;
     tpv0 = t[0]; tpv1 = t[1]; tpv2 = t[2]; tpv3 = t[3];
   }
-  // 2) LDS
-  for (int i = tid; i < 2 * NBUF; i += blockDim.x) smem[i] = 0.0f;
+  // 2) LDS (buf0's region cells are filled from U0 below)
+  cs_zero_lds(buf0, buf1, g, ((g.RH + R - 1) / R) * R, NBUF);
   WT_TS(4);
   if (tid < a.nsteps) xs[tid] = xsv;
   if (tid < nsp) { tb_sp[3 * tid] = spv0; tb_sp[3 * tid + 1] = spv1; tb_sp[3 * tid + 2] = spv2; }
@@ -498,8 +521,8 @@ __device__ __forceinline__ void adj_cs_body(const AdjArgs &a, const int w) {
     const int32_t *e = a.spx + (td[8] + tid) * SPX_W;
     e0 = e[0]; e1 = e[1]; e2 = e[2]; e3 = e[3]; e4 = e[4]; e5 = e[5];
   }
-  // 2) LDS
-  for (int i = tid; i < 2 * NBUF; i += blockDim.x) smem[i] = 0.0f;
+  // 2) LDS (buf0's region cells are filled from the lambda pair below)
+  cs_zero_lds(buf0, buf1, g, ((g.RH + R - 1) / R) * R, NBUF);
   if (tid < nsp) {
     int *t = tb_sp + 6 * tid;
     t[0] = e0; t[1] = e1; t[2] = e2; t[3] = e3; t[4] = e4; t[5] = e5;
