@@ -55,6 +55,42 @@ def test_summarize_pmc_calibrates_on_largest_copy(tmp_path, monkeypatch):
     assert bench.traffic("unit_test", "wt::k_adj_cs<1024, 8>", {"a": 2}) is None
 
 
+def test_summarize_pmc_tail_dispatches(tmp_path, monkeypatch):
+    """With `tail_dispatches` in the bench's meta file a kernel's summary
+    covers only its last n dispatches (in dispatch order), and a kernel with
+    n = 0 is left out."""
+    cp = "__amd_rocclr_copyBuffer"
+    ka = "void wt::k_adj_cs_batch<1024, 10>(wt::BatchAdj)"
+    kf = "void wt::k_fwd_cs_batch<1024, 10, 0, false>(wt::BatchFwd)"
+    # rows out of dispatch order: ids 0..5; the adjoint's last two are 300, 500
+    rows = [(cp, 512.0), (ka, 100.0), (ka, 500.0), (kf, 7.0), (ka, 300.0)]
+    order = [0, 1, 5, 2, 3]
+
+    def write(d, counter, vals):
+        os.makedirs(d, exist_ok=True)
+        with open(os.path.join(d, "run_counter_collection.csv"), "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=["Dispatch_Id", "Kernel_Name", "Counter_Name",
+                                              "Counter_Value"])
+            w.writeheader()
+            for i, (k, v) in zip(order, vals):
+                w.writerow({"Dispatch_Id": i, "Kernel_Name": k, "Counter_Name": counter,
+                            "Counter_Value": v})
+
+    write(tmp_path / "f", "FETCH_SIZE", rows)
+    write(tmp_path / "w", "WRITE_SIZE", [(k, v / 2) for k, v in rows])
+    (tmp_path / "f" / "bench_meta.json").write_text(json.dumps({
+        "tuning": {"a": 1}, "workload": "unit_tail",
+        "tail_dispatches": {"wt::k_adj_cs_batch<1024, 10>": 2,
+                            "wt::k_fwd_cs_batch<1024, 10, 0, false>": 0}}))
+    monkeypatch.setattr(bench, "traffic_file", lambda w: str(tmp_path / f"traffic_{w}.json"))
+    bench.summarize_pmc(str(tmp_path / "f"), str(tmp_path / "w"), 2 ** 20)
+    d = json.loads((tmp_path / "traffic_unit_tail.json").read_text())
+    k = d["kernels"]["wt::k_adj_cs_batch<1024, 10>"]
+    assert k["dispatches"] == 2
+    assert k["fetch_kib"] == pytest.approx((300.0 + 500.0) / 2)
+    assert "wt::k_fwd_cs_batch<1024, 10, 0, false>" not in d["kernels"]
+
+
 def test_roofline_takes_the_binding_unit(tmp_path, monkeypatch):
     """roofline(): HBM fraction from the counted bytes, VALU fraction from
     SQ_INSTS_VALU x 64 lanes against 256 CUs x 64 lanes x 2.4 GHz; `frac`
