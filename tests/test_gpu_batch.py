@@ -191,3 +191,27 @@ def test_batch_instance_order(parts, K, cuda):
         assert np.array_equal(y.cpu().numpy(), yo), inst.length
         assert rel(gc.cpu().numpy(), gco) < 1e-4, inst.length
         assert rel(gx.cpu().numpy(), gxo) < 1e-4, inst.length
+
+
+def test_batch_forward_only_without_saved_workspace(cuda):
+    """InstanceBatch(grad=False) runs the forward with no saved workspace and
+    gives the probe series of the gradient-carrying batch bit for bit; the
+    reverse sweep is refused."""
+    items = _instances(seed=5)
+    cs = [torch.from_numpy(it[1]).to(cuda) for it in items]
+    xs = [torch.from_numpy(it[2]).to(cuda) for it in items]
+    ys = []
+    for grad in (True, False):
+        bt = InstanceBatch([it[0] for it in items], cuda, checkpoint_every=300, grad=grad)
+        print("case forward-only", grad, flush=True)
+        assert (bt.saved is None) == (not grad)
+        bt.set_inputs(cs, xs)
+        bt.forward(with_grad=grad)
+        torch.cuda.synchronize()
+        ys.append([y.clone() for (y, _, _) in bt.outputs()])
+        if not grad:
+            with pytest.raises(RuntimeError):
+                bt.fwd_adj()
+        del bt
+    for a, b in zip(*ys):
+        assert torch.equal(a, b)

@@ -42,10 +42,12 @@ class InstanceBatch:
     Per group g (instances longest first, or in the given order with
     `longest_first` False): c (B,Nx,Ny), x (B,T) with T the
     longest length of the batch (zero-padded), ybar (B,T,P), outputs y
-    (B,T,P), grad_c (B,Nx,Ny), grad_x (B,T)."""
+    (B,T,P), grad_c (B,Nx,Ny), grad_x (B,T).  With `grad` False the batch
+    runs forward only and holds no saved workspace (checkpoints and the
+    forward's record, the bulk of the memory of a one-segment plan)."""
 
     def __init__(self, instances, device, tuning=None, checkpoint_every=64,
-                 longest_first=True):
+                 longest_first=True, grad=True):
         self.device = torch.device(device)
         self.instances = list(instances)
         groups = {}
@@ -105,7 +107,11 @@ class InstanceBatch:
         _lib.check(L.wt_batch_workspace_bytes(h, ctypes.byref(s0), ctypes.byref(t0)),
                    "wt_batch_workspace_bytes")
         dev = self.device
-        self.saved = torch.empty(max(s0.value, 1), dtype=torch.uint8, device=dev)
+        # the saved workspace (checkpoints, the forward's record) only when
+        # gradients are wanted: forward-only batches run without it
+        self.grad = bool(grad)
+        self.saved = (torch.empty(max(s0.value, 1), dtype=torch.uint8, device=dev)
+                      if self.grad else None)
         self.temp = torch.empty(max(t0.value, 1), dtype=torch.uint8, device=dev)
         # per-group buffers
         self.buf = []
@@ -170,17 +176,27 @@ class InstanceBatch:
     def _stream(self):
         return _lib.stream_ptr(self.device)
 
+    def _need_grad(self):
+        if not self.grad:
+            raise RuntimeError("InstanceBatch(grad=False) holds no saved workspace: forward "
+                               "only (forward(with_grad=False))")
+
     def forward(self, with_grad=True):
+        if with_grad:
+            self._need_grad()
         L = _lib.lib()
-        _lib.check(L.wt_batch_fwd(self.handle, self.io, int(with_grad), self.saved.data_ptr(),
+        saved = self.saved.data_ptr() if self.saved is not None else None
+        _lib.check(L.wt_batch_fwd(self.handle, self.io, int(with_grad), saved,
                                   self.temp.data_ptr(), self._stream()), "wt_batch_fwd")
 
     def backward(self):
+        self._need_grad()
         L = _lib.lib()
         _lib.check(L.wt_batch_adj(self.handle, self.io, self.saved.data_ptr(),
                                   self.temp.data_ptr(), self._stream()), "wt_batch_adj")
 
     def fwd_adj(self):
+        self._need_grad()
         L = _lib.lib()
         _lib.check(L.wt_batch_fwd_adj(self.handle, self.io, self.saved.data_ptr(),
                                       self.temp.data_ptr(), self._stream()), "wt_batch_fwd_adj")
