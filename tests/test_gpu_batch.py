@@ -128,11 +128,14 @@ def test_batch_parts_match_single(parts, cuda):
         assert rel(gxp.cpu().numpy(), gxo) < 1e-4, inst.length
 
 
-def test_batch_parts_graph_replay(cuda):
+@pytest.mark.parametrize("K", [64, 300])
+def test_batch_parts_graph_replay(K, cuda):
     """A captured graph of a three-part batch (fork and join events, part
-    streams) replays the eager result, with eager work between replays."""
+    streams) replays the eager result, with eager work between replays; K =
+    300 is one segment (the forward's packed record, no recompute)."""
     items = _instances(seed=3)
-    bt = InstanceBatch([it[0] for it in items], cuda, tuning={"streams": 3})
+    bt = InstanceBatch([it[0] for it in items], cuda, tuning={"streams": 3},
+                       checkpoint_every=K)
     assert bt.tuning[0]["streams"] == 3
     xs = [torch.from_numpy(it[2]).to(cuda) for it in items]
     ybars = [-torch.ones((it[0].length, 1), device=cuda) for it in items]
